@@ -1,0 +1,231 @@
+"""Benchmark of the 8th-order acoustic time step on one MI355X.
+
+Workload (one "step" = one launch of the time-step kernel over the grid):
+acoustic wave, 512^3 cells (504^3 interior), h = 10 m, dt = 1e-3 s, FP32,
+seeded 16-layer velocity model v ~ U[1500, 4000] m/s per layer with +-5 %
+per-cell noise, centred Gaussian pulse (sigma = 3 cells) in p and p_prev.
+Inputs are resident in HBM before the timed region; K steps are timed with
+HIP events on the stream the kernels are launched on.
+
+Prints ONE JSON line.  ``value`` is interior points updated per second (Gpts/s)
+of the tiled kernel; ``roofline`` prices it against 8 TB/s with the algorithmic
+16 B / point (read p, p_prev, vel2; write p_next); ``cpu_baseline`` times the
+reference's own CPU path (oracle/_ref, OpenMP on the host cores) on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gpts/s @512³: FP32 plain vs BF16-store (rel-L2≤1e-3 vs FP64); grad time"
+HBM_PEAK_GBS = 8000.0
+REF_GPTS_RTX4050 = 11.17  # reference naive FP32 at 512^3: 11.463 ms/step (docs/CASE_STUDY.md:86)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512, help="cells per axis (halo included)")
+    ap.add_argument("--variant", default="tiled", choices=["tiled", "naive", "tiled_exact"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the side measurements")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path has no halo exchange here: one GPU, one grid).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import stencil_solver_amd as ss
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    n = args.n
+    shape = (n, n, n)
+    h, dt = np.float32(10.0), np.float32(1e-3)
+    grid = ss.GridDims.cube(n, h, dt)
+    interior = (n - 8) ** 3
+
+    t0 = time.time()
+    p0_h = ss.host_gaussian(shape, 0.5, 0.5, 0.5, 3.0)
+    vel2_h = ss.host_layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.05, 2)
+    p0 = torch.from_numpy(p0_h).to(dev)
+    vel2 = torch.from_numpy(vel2_h).to(dev)
+    print(f"[bench] inputs {n}^3 ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def launcher(variant):
+        if variant == "naive":
+            return lambda c, p, v, o: ss.launch_naive(c, p, v, o, grid)
+        if variant == "tiled_exact":
+            return lambda c, p, v, o: ss.launch_tiled_ex(c, p, v, o, grid, exact=True)
+        return lambda c, p, v, o: ss.launch_tiled(c, p, v, o, grid)
+
+    def timed_run(variant, steps, warmup):
+        fn = launcher(variant)
+        bufs = [p0.clone(), p0.clone(), torch.zeros_like(p0)]
+        i = 0
+        for _ in range(warmup):
+            fn(bufs[(i + 1) % 3], bufs[i], vel2, bufs[(i + 2) % 3])
+            i = (i + 1) % 3
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()  # the current stream: the one the launchers use
+        for _ in range(steps):
+            fn(bufs[(i + 1) % 3], bufs[i], vel2, bufs[(i + 2) % 3])
+            i = (i + 1) % 3
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        out = bufs[(i + 1) % 3]
+        del bufs
+        return ms, out
+
+    ms, out = timed_run(args.variant, args.steps, args.warmup)
+    gpts = interior / (ms * 1e-3) / 1e9
+    alg_bytes = 16.0 * interior
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    print(f"[bench] {args.variant}: {ms:.4f} ms/step, {gpts:.1f} Gpts/s, {achieved:.0f} GB/s",
+          file=sys.stderr, flush=True)
+
+    result = {
+        "metric": METRIC,
+        "value": round(gpts, 3),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(gpts / REF_GPTS_RTX4050, 2),
+        "baseline": {"value": REF_GPTS_RTX4050, "unit": "Gpts/s",
+                     "what": "reference naive FP32 kernel, 512^3, RTX 4050 Laptop (11.463 ms/step)"},
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"acoustic {n}^3 FP32 leapfrog, 8th-order Laplacian, 16-layer +-5% velocity, "
+                               f"Gaussian sigma=3, h=10 m, dt=1e-3 s",
+                   "grid": n, "interior_points": interior, "kernel": args.variant},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+    }
+
+    # Measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+    # scaled by a known-size copy), committed under profiles/ for this kernel.
+    tfile = ROOT / "profiles" / f"traffic_wave_{args.variant}_{n}.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text())
+        result["roofline"]["traffic"] = t.get("bytes_per_launch")
+
+    if not args.no_extras:
+        extras = {}
+        # Same steps through the naive (reference-association) kernel: the
+        # measured run's result must agree with it (tolerance 1e-5 rel-L2).
+        ms_n, out_n = timed_run("naive", args.steps, 0)
+        rel = (torch.linalg.vector_norm((out - out_n).double()) /
+               torch.linalg.vector_norm(out_n.double())).item()
+        extras["naive_gpts"] = round(interior / (ms_n * 1e-3) / 1e9, 2)
+        extras["rel_l2_vs_naive_after_steps"] = rel
+        del out_n
+        ms_e, _ = timed_run("tiled_exact", min(args.steps, 200), 5)
+        extras["tiled_exact_gpts"] = round(interior / (ms_e * 1e-3) / 1e9, 2)
+        # Heat step (dedicated single-level kernel): 8 B / point.
+        a, b = p0.clone(), torch.zeros_like(p0)
+        hgrid = ss.GridDims.cube(n, h, np.float32(np.sqrt(np.float64(5.0))))
+        for _ in range(5):
+            ss.heat_step(a, b, 1.0, hgrid)
+            a, b = b, a
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(200):
+            ss.heat_step(a, b, 1.0, hgrid)
+            a, b = b, a
+        e1.record()
+        torch.cuda.synchronize()
+        hms = e0.elapsed_time(e1) / 200
+        extras["heat_tiled_gbs"] = round(8.0 * interior / (hms * 1e-3) / 1e9, 1)
+        extras["heat_tiled_ms"] = round(hms, 5)
+        del a, b
+        # STREAM copy of one field-sized buffer: achievable HBM bandwidth.
+        src = torch.ones(n * n * n, device=dev)
+        dst = torch.empty_like(src)
+        L = ss.lib()
+        stream = torch.cuda.current_stream().cuda_stream
+        for _ in range(3):
+            L.stencil_stream_f32(0, dst.data_ptr(), src.data_ptr(), src.numel(), stream)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            L.stencil_stream_f32(0, dst.data_ptr(), src.data_ptr(), src.numel(), stream)
+        e1.record()
+        torch.cuda.synchronize()
+        cms = e0.elapsed_time(e1) / 20
+        extras["stream_copy_gbs"] = round(8.0 * src.numel() / (cms * 1e-3) / 1e9, 1)
+        del src, dst
+        result["extras"] = extras
+        print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(p0_h, vel2_h, h, dt, interior, args.cpu_seconds)
+
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(p0_h, vel2_h, h, dt, interior, seconds):
+    """The reference CPU path (src/kernels/stencil_naive.cu:136-170, OpenMP)
+    built from its sources into oracle/_ref/libref_v3.so, timed like
+    SolverBase::run in CPU mode on a bounded number of steps of the same
+    workload; the oracle's C restatement if that build is absent."""
+    from oracle.lib import Oracle, RefLib, ref_available
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if ref_available("v3"):
+        ref = RefLib("v3")
+        kind = "reference"
+        one = ref.time_wave_steps(p0_h, vel2_h, h, dt, 0, 1)
+        n_steps = int(max(2, min(60, seconds * 1e3 / max(one, 1e-3))))
+        ms = ref.time_wave_steps(p0_h, vel2_h, h, dt, 1, n_steps)
+    else:
+        import numpy as np
+        o = Oracle()
+        kind = "port"
+        bufs = [p0_h.copy(), p0_h.copy(), np.zeros_like(p0_h)]
+        t = time.perf_counter()
+        o.wave_step(bufs[1], bufs[0], vel2_h, bufs[2], h, dt)
+        one = (time.perf_counter() - t) * 1e3
+        n_steps = int(max(2, min(60, seconds * 1e3 / max(one, 1e-3))))
+        t = time.perf_counter()
+        i = 1
+        for _ in range(n_steps):
+            o.wave_step(bufs[(i + 1) % 3], bufs[i], vel2_h, bufs[(i + 2) % 3], h, dt)
+            i = (i + 1) % 3
+        ms = (time.perf_counter() - t) * 1e3 / n_steps
+    gpts = interior / (ms * 1e-3) / 1e9
+    print(f"[bench] cpu baseline ({kind}, {cores} threads): {ms:.1f} ms/step, {gpts:.3f} Gpts/s",
+          file=sys.stderr, flush=True)
+    return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": cores, "kind": kind,
+            "ms_per_step": round(ms, 2),
+            "sample": f"{n_steps} timed steps (after 1 warmup) of the same 512^3 workload through the reference "
+                      f"launch_naive CPU path (OpenMP, {cores} threads, no FTZ/DAZ)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * include/stencil_capi.h -- the C-ABI of the MI355X stencil library
+ * (libstencil_hip.so, built from stencil_solver_amd/csrc/).
+ *
+ * Plain pointers and sizes only.  Every launcher is asynchronous on the HIP
+ * stream it is given (NULL = the default stream), never synchronises, never
+ * allocates, never exits, and returns STENCIL_OK or an error code
+ * (stencil_strerror() names it).  Buffers are caller-owned device pointers in
+ * the reference layout: x fastest, idx = z*nx*ny + y*nx + x, dims include a
+ * radius-4 halo on every face that the launchers never write
+ * (include/stencil/types.hpp:110-136 of the reference).  p_next must not
+ * alias an input.
+ */
+#ifndef STENCIL_CAPI_H
+#define STENCIL_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STENCIL_CAPI_RADIUS 4
+
+/* Grid descriptor, the C form of stencil::GridDims (reference
+ * include/stencil/types.hpp:115-140).  Spacings and dt are carried in double;
+ * the f32 launchers narrow them to float first, so a float GridDims passes
+ * through unchanged. */
+typedef struct stencil_grid {
+    uint64_t nx, ny, nz; /* total cells, halos included */
+    double dx, dy, dz;   /* spacing [m] */
+    double dt;           /* time step [s] */
+} stencil_grid;
+
+/* KernelVariant (reference include/stencil/config.hpp:25-29). */
+enum stencil_variant {
+    STENCIL_VARIANT_NAIVE = 0,    /* coalesced global-memory kernel */
+    STENCIL_VARIANT_TILED = 1,    /* LDS plane + z register queue */
+    STENCIL_VARIANT_MULTIGPU = 2  /* one GPU here: same as TILED (kernels.cuh:78-81) */
+};
+
+enum stencil_status {
+    STENCIL_OK = 0,
+    STENCIL_ERR_INVALID = 1,     /* bad dims / null pointer / bad enum */
+    STENCIL_ERR_HIP = 2,         /* a HIP runtime call failed */
+    STENCIL_ERR_UNSUPPORTED = 3, /* shape the kernel cannot take */
+    STENCIL_ERR_NOMEM = 4
+};
+
+const char* stencil_strerror(int status);
+/* The HIP error code behind the last STENCIL_ERR_HIP on this thread. */
+int stencil_last_hip_error(void);
+
+/* Replaces upload_fd_coefficients() (reference include/stencil/kernels.cuh:33,
+ * src/kernels/stencil_naive.cu:38-46).  The coefficients are compile-time
+ * constants of the gfx950 kernels, so this only checks that a device is
+ * usable; kept so callers written against the reference still link. */
+int stencil_upload_fd_coefficients(void);
+
+/* Replaces launch_naive (reference include/stencil/kernels.cuh:42-43,
+ * src/kernels/stencil_naive.cu:92-114): one acoustic leapfrog step
+ *   p_next = 2 p_cur - p_prev + dt^2 vel2 Lap(p_cur)
+ * over the interior, in the reference's exact association (bit-identical to
+ * its CPU path built without FMA contraction). */
+int stencil_launch_naive_f32(const float* p_cur, const float* p_prev, const float* vel2,
+                             float* p_next, const stencil_grid* grid, void* hip_stream);
+/* Replaces launch_tiled (reference include/stencil/kernels.cuh:54-55,
+ * src/kernels/stencil_tiled.cu:147-168): same step, LDS-staged plane + z
+ * register queue, FMA-fused weights (within 1e-5 rel-L2 of the reference). */
+int stencil_launch_tiled_f32(const float* p_cur, const float* p_prev, const float* vel2,
+                             float* p_next, const stencil_grid* grid, void* hip_stream);
+/* Replaces launch(KernelVariant, ...) (reference include/stencil/kernels.cuh:68-83). */
+int stencil_launch_f32(int variant, const float* p_cur, const float* p_prev, const float* vel2,
+                       float* p_next, const stencil_grid* grid, void* hip_stream);
+
+/* The STENCIL_FP64 build of the same three (reference types.hpp:33-37);
+ * the FP64 oracle run. */
+int stencil_launch_naive_f64(const double* p_cur, const double* p_prev, const double* vel2,
+                             double* p_next, const stencil_grid* grid, void* hip_stream);
+int stencil_launch_tiled_f64(const double* p_cur, const double* p_prev, const double* vel2,
+                             double* p_next, const stencil_grid* grid, void* hip_stream);
+int stencil_launch_f64(int variant, const double* p_cur, const double* p_prev,
+                       const double* vel2, double* p_next, const stencil_grid* grid,
+                       void* hip_stream);
+
+/* Replaces the heat step of HeatDiffusionSolver::step_impl (reference
+ * src/solver/heat_diffusion.cpp:103-122: wave kernel with vel2 = alpha and
+ * p_prev = p_cur, then a full-field D2D copy) with one single-level kernel:
+ *   t_next = t + (dt^2 alpha) Lap(t)
+ * where grid->dt is the heat solver's dt_eff = (float)sqrt((double)dt)
+ * (heat_diffusion.cpp:86-87).  Bit-identical to the reference for
+ * STENCIL_VARIANT_NAIVE; TILED uses the fused-weight form. */
+int stencil_heat_step_f32(const float* t, float* t_next, float alpha, const stencil_grid* grid,
+                          int variant, void* hip_stream);
+
+/* The STENCIL_FP64 heat step (naive kernel for every variant). */
+int stencil_heat_step_f64(const double* t, double* t_next, double alpha, const stencil_grid* grid,
+                          int variant, void* hip_stream);
+
+/* Algorithmic HBM bytes one launch moves (reference types.hpp:209-212:
+ * 4 fields x interior x sizeof(real_t) for the wave step; 2 fields for heat). */
+double stencil_wave_bytes_per_step(const stencil_grid* grid, int bytes_per_value);
+double stencil_heat_bytes_per_step(const stencil_grid* grid, int bytes_per_value);
+
+/* ---- Extensions (no counterpart in the reference) ---- */
+
+/* The tiled step with an explicit launch shape, for the autotuner and for
+ * measurement: exact != 0 selects the reference association; ty in
+ * {16, 24, 32} rows per tile (0 = default), n_chunks z-chunks per column
+ * (0 = automatic), xcd_remap 1/0/-1 = on/off/automatic.  Returns
+ * STENCIL_ERR_UNSUPPORTED when nx % 4 != 0 or a base is not 16-byte aligned
+ * (stencil_launch_tiled_f32 then runs the naive kernel instead). */
+int stencil_launch_tiled_ex_f32(const float* p_cur, const float* p_prev, const float* vel2,
+                                float* p_next, const stencil_grid* grid, int exact, int ty,
+                                int n_chunks, int xcd_remap, void* hip_stream);
+int stencil_heat_step_ex_f32(const float* t, float* t_next, float alpha, const stencil_grid* grid,
+                             int exact, int ty, int n_chunks, int xcd_remap, void* hip_stream);
+
+/* HBM calibration (the reference's tests/perf/bench_stream.cu:53-101):
+ * op 0 = copy src -> dst, 1 = read src (dst receives at most 2048 partial
+ * sums), 2 = write dst := 1.  n floats, n % 4 == 0, 16-byte aligned. */
+int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* hip_stream);
+
+/* Seeded host-side input generators (OpenMP), the same values on every
+ * platform: see stencil::init_gaussian / init_layered_noise_vel2 /
+ * init_noise_cube in include/stencil/grid.hpp. */
+int stencil_host_init_gaussian_f32(float* host, uint64_t nx, uint64_t ny, uint64_t nz, float cx,
+                                   float cy, float cz, float sigma);
+int stencil_host_layered_noise_vel2_f32(float* host, uint64_t nx, uint64_t ny, uint64_t nz,
+                                        int n_layers, float vmin, float vmax, float noise,
+                                        uint64_t seed);
+int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t nz,
+                                uint64_t side, float hot, uint64_t seed);
+
+/* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)
+ * for callers that cannot use the C++ API. ---- */
+typedef struct stencil_config {
+    uint64_t nx, ny, nz;
+    double dx, dy, dz;
+    double dt; /* 0 = CFL-derived (Config::make_grid) */
+    int steps, warmup_steps;
+    double velocity, diffusivity;
+    double pulse_cx, pulse_cy, pulse_cz, pulse_sigma;
+    int physics; /* 0 wave, 1 heat */
+    int variant; /* enum stencil_variant */
+    int device;
+    int vmodel; /* 0 constant, 1 layered-noise */
+    int n_layers;
+    double vmin, vmax, vnoise;
+    int ic; /* 0 gaussian, 1 noise-cube */
+    uint64_t cube_side;
+    double cube_value;
+    uint64_t seed;
+    int fp64; /* 0: float fields (real_t = float), 1: the STENCIL_FP64 solver */
+} stencil_config;
+
+typedef struct stencil_perf {
+    double time_ms, bandwidth_gbs, gflops, model_bandwidth_gbs;
+    uint64_t grid_size;
+    char label[32];
+} stencil_perf;
+
+typedef struct stencil_solver stencil_solver;
+
+/* Fills the reference Config defaults (config.hpp:34-70). */
+void stencil_config_default(stencil_config* cfg);
+int stencil_solver_create(const stencil_config* cfg, stencil_solver** out);
+int stencil_solver_step(stencil_solver* s);
+int stencil_solver_run(stencil_solver* s, int n_steps, int warmup, stencil_perf* out);
+/* Copies the newest time level into host memory of nx*ny*nz values of the
+ * solver's type (bytes must match) and synchronises. */
+int stencil_solver_download(stencil_solver* s, void* host, uint64_t bytes);
+int stencil_solver_dims(const stencil_solver* s, stencil_grid* out);
+int stencil_solver_name(const stencil_solver* s, char* buf, uint64_t len);
+void stencil_solver_destroy(stencil_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STENCIL_CAPI_H */

@@ -1,0 +1,307 @@
+/*
+ * oracle/stencil_oracle.c -- CPU restatement of the reference stencil path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load this library; the product never
+ * does.  Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ *
+ * Each function restates the reference function cited in its comment, in the
+ * same loop order and floating-point association, so that with contraction
+ * off it is bit-identical to the reference CPU path built with
+ * -ffp-contract=off (checked against oracle/_ref and the committed fixtures
+ * in tests/golden/).
+ */
+#include "stencil_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+#define R ORACLE_R
+
+/* include/stencil/types.hpp:175-181 -- static_cast<real_t> of the double
+ * rationals. */
+void oracle_fd_coeffs_f32(float c[5]) {
+    c[0] = (float)(-205.0 / 72.0);
+    c[1] = (float)(8.0 / 5.0);
+    c[2] = (float)(-1.0 / 5.0);
+    c[3] = (float)(8.0 / 315.0);
+    c[4] = (float)(-1.0 / 560.0);
+}
+
+void oracle_fd_coeffs_f64(double c[5]) {
+    c[0] = -205.0 / 72.0;
+    c[1] = 8.0 / 5.0;
+    c[2] = -1.0 / 5.0;
+    c[3] = 8.0 / 315.0;
+    c[4] = -1.0 / 560.0;
+}
+
+/* src/kernels/stencil_naive.cu:136-170 (CPU launch_naive).  The scalars are
+ * formed exactly as at :145-148: inv = 1/(h*h), dt2 = dt*dt, in real_t. */
+#define ORACLE_WAVE_STEP(T, NAME, COEFFS)                                                     \
+    void NAME(const T* p_cur, const T* p_prev, const T* vel2, T* p_next, size_t nx, size_t ny, \
+              size_t nz, T dx, T dy, T dz, T dt) {                                            \
+        T c_fd[5];                                                                            \
+        COEFFS(c_fd);                                                                         \
+        const int NX = (int)nx, NY = (int)ny, NZ = (int)nz;                                   \
+        const T inv_dx2 = (T)1 / (dx * dx);                                                   \
+        const T inv_dy2 = (T)1 / (dy * dy);                                                   \
+        const T inv_dz2 = (T)1 / (dz * dz);                                                   \
+        const T dt2 = dt * dt;                                                                \
+        const ptrdiff_t sy = NX;                                                              \
+        const ptrdiff_t sz = (ptrdiff_t)NX * NY;                                              \
+        _Pragma("omp parallel for collapse(3) schedule(static)")                              \
+        for (int iz = R; iz < NZ - R; ++iz)                                                   \
+            for (int iy = R; iy < NY - R; ++iy)                                               \
+                for (int ix = R; ix < NX - R; ++ix) {                                         \
+                    const ptrdiff_t c = iz * sz + iy * sy + ix;                               \
+                    T lap_x = c_fd[0] * p_cur[c] * inv_dx2;                                   \
+                    T lap_y = c_fd[0] * p_cur[c] * inv_dy2;                                   \
+                    T lap_z = c_fd[0] * p_cur[c] * inv_dz2;                                   \
+                    for (int r = 1; r <= R; ++r) {                                            \
+                        lap_x += c_fd[r] * (p_cur[c + r] + p_cur[c - r]) * inv_dx2;           \
+                        lap_y += c_fd[r] * (p_cur[c + r * sy] + p_cur[c - r * sy]) * inv_dy2; \
+                        lap_z += c_fd[r] * (p_cur[c + r * sz] + p_cur[c - r * sz]) * inv_dz2; \
+                    }                                                                         \
+                    p_next[c] = (T)2 * p_cur[c] - p_prev[c] +                                 \
+                                dt2 * vel2[c] * (lap_x + lap_y + lap_z);                      \
+                }                                                                             \
+    }
+
+ORACLE_WAVE_STEP(float, oracle_wave_step_f32, oracle_fd_coeffs_f32)
+ORACLE_WAVE_STEP(double, oracle_wave_step_f64, oracle_fd_coeffs_f64)
+
+/* include/stencil/config.hpp:75-87 -- dt = CFL_LIMIT * min(dx,dy,dz) / v,
+ * CFL_LIMIT = static_cast<real_t>(0.45) (types.hpp:139). */
+float oracle_make_grid_dt_f32(float dx, float dy, float dz, float velocity) {
+    float min_h = dx;
+    if (dy < min_h) min_h = dy;
+    if (dz < min_h) min_h = dz;
+    const float cfl = (float)0.45;
+    return cfl * min_h / velocity;
+}
+
+/* src/solver/heat_diffusion.cpp:86-87 -- heat_dims_.dt =
+ * std::sqrt(static_cast<double>(dt)) narrowed to real_t. */
+float oracle_heat_dt_eff_f32(float dt) { return (float)sqrt((double)dt); }
+
+/* src/solver/heat_diffusion.cpp:103-122: the wave kernel with vel2 == alpha
+ * everywhere (:79-81) and p_prev == p_cur (the copy at :116-118), using the
+ * dims with dt_eff.  Written as the wave step with p_prev = t and a constant
+ * vel2, which is the same arithmetic. */
+void oracle_heat_step_f32(const float* t, float* t_next, size_t nx, size_t ny, size_t nz,
+                          float dx, float dy, float dz, float dt_eff, float alpha) {
+    float c_fd[5];
+    oracle_fd_coeffs_f32(c_fd);
+    const int NX = (int)nx, NY = (int)ny, NZ = (int)nz;
+    const float inv_dx2 = 1.0f / (dx * dx);
+    const float inv_dy2 = 1.0f / (dy * dy);
+    const float inv_dz2 = 1.0f / (dz * dz);
+    const float dt2 = dt_eff * dt_eff;
+    const ptrdiff_t sy = NX;
+    const ptrdiff_t sz = (ptrdiff_t)NX * NY;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int iz = R; iz < NZ - R; ++iz)
+        for (int iy = R; iy < NY - R; ++iy)
+            for (int ix = R; ix < NX - R; ++ix) {
+                const ptrdiff_t c = iz * sz + iy * sy + ix;
+                float lap_x = c_fd[0] * t[c] * inv_dx2;
+                float lap_y = c_fd[0] * t[c] * inv_dy2;
+                float lap_z = c_fd[0] * t[c] * inv_dz2;
+                for (int r = 1; r <= R; ++r) {
+                    lap_x += c_fd[r] * (t[c + r] + t[c - r]) * inv_dx2;
+                    lap_y += c_fd[r] * (t[c + r * sy] + t[c - r * sy]) * inv_dy2;
+                    lap_z += c_fd[r] * (t[c + r * sz] + t[c - r * sz]) * inv_dz2;
+                }
+                t_next[c] = 2.0f * t[c] - t[c] + dt2 * alpha * (lap_x + lap_y + lap_z);
+            }
+}
+
+/* include/stencil/grid.hpp:57-78. */
+void oracle_init_gaussian_f32(float* g, size_t nx, size_t ny, size_t nz, float cx, float cy,
+                              float cz, float sigma) {
+    const float icx = cx * (float)(nx - 1);
+    const float icy = cy * (float)(ny - 1);
+    const float icz = cz * (float)(nz - 1);
+    const float inv2s2 = 1.0f / (2.0f * sigma * sigma);
+    for (size_t iz = R; iz < nz - R; ++iz)
+        for (size_t iy = R; iy < ny - R; ++iy)
+            for (size_t ix = R; ix < nx - R; ++ix) {
+                const float dx = (float)ix - icx;
+                const float dy = (float)iy - icy;
+                const float dz = (float)iz - icz;
+                g[iz * nx * ny + iy * nx + ix] = expf(-(dx * dx + dy * dy + dz * dz) * inv2s2);
+            }
+}
+
+void oracle_init_gaussian_f64(double* g, size_t nx, size_t ny, size_t nz, double cx, double cy,
+                              double cz, double sigma) {
+    const double icx = cx * (double)(nx - 1);
+    const double icy = cy * (double)(ny - 1);
+    const double icz = cz * (double)(nz - 1);
+    const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+    for (size_t iz = R; iz < nz - R; ++iz)
+        for (size_t iy = R; iy < ny - R; ++iy)
+            for (size_t ix = R; ix < nx - R; ++ix) {
+                const double dx = (double)ix - icx;
+                const double dy = (double)iy - icy;
+                const double dz = (double)iz - icz;
+                g[iz * nx * ny + iy * nx + ix] = exp(-(dx * dx + dy * dy + dz * dz) * inv2s2);
+            }
+}
+
+/* include/stencil/grid.hpp:83-85. */
+void oracle_init_constant_velocity_f32(float* g, size_t n, float v) {
+    const float v2 = v * v;
+    for (size_t i = 0; i < n; ++i) g[i] = v2;
+}
+
+/* include/stencil/grid.hpp:91-99. */
+void oracle_init_layered_velocity_f32(float* g, size_t nx, size_t ny, size_t nz, float v_top,
+                                      float v_bot) {
+    for (size_t iz = 0; iz < nz; ++iz)
+        for (size_t iy = 0; iy < ny; ++iy)
+            for (size_t ix = 0; ix < nx; ++ix) {
+                const float v = (iz < nz / 2) ? v_top : v_bot;
+                g[iz * nx * ny + iy * nx + ix] = v * v;
+            }
+}
+
+/* src/utils/grid.cpp:86-97 -- RMS over every cell, halos included. */
+double oracle_l2_error_f32(const float* a, const float* b, size_t n) {
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double d = (double)a[i] - (double)b[i];
+        sum += d * d;
+    }
+    return sqrt(sum / (double)n);
+}
+
+/* src/utils/grid.cpp:102-115. */
+double oracle_max_abs_error_f32(const float* a, const float* b, size_t n) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double d = fabs((double)a[i] - (double)b[i]);
+        if (d > m) m = d;
+    }
+    return m;
+}
+
+/* src/solver/solver_base.cpp:28-62 (both levels start at the IC), :65-108
+ * (every step is step_impl), src/solver/wave_prop.cpp:67-75 (launch, then
+ * rotate), include/stencil/grid.hpp:144-153 (prev=buf[i], cur=buf[i+1],
+ * next=buf[i+2]) and :174-176 (download returns cur).  All three levels are
+ * zeroed first, which is what keeps the Dirichlet halo at zero
+ * (grid.hpp:117,156-160). */
+#define ORACLE_WAVE_RUN(T, NAME, STEP)                                                         \
+    void NAME(const T* p0, const T* vel2, T* buf0, T* buf1, T* buf2, T* out, size_t nx,         \
+              size_t ny, size_t nz, T dx, T dy, T dz, T dt, int n_steps) {                     \
+        const size_t n = nx * ny * nz;                                                         \
+        T* buf[3] = {buf0, buf1, buf2};                                                        \
+        memset(buf2, 0, n * sizeof(T));                                                        \
+        memcpy(buf0, p0, n * sizeof(T));                                                       \
+        memcpy(buf1, p0, n * sizeof(T));                                                       \
+        int idx = 0;                                                                           \
+        for (int s = 0; s < n_steps; ++s) {                                                    \
+            STEP(buf[(idx + 1) % 3], buf[idx], vel2, buf[(idx + 2) % 3], nx, ny, nz, dx, dy,   \
+                 dz, dt);                                                                      \
+            idx = (idx + 1) % 3;                                                               \
+        }                                                                                      \
+        memcpy(out, buf[(idx + 1) % 3], n * sizeof(T));                                        \
+    }
+
+ORACLE_WAVE_RUN(float, oracle_wave_run_f32, oracle_wave_step_f32)
+ORACLE_WAVE_RUN(double, oracle_wave_run_f64, oracle_wave_step_f64)
+
+/* src/solver/heat_diffusion.cpp:103-122: step into `next`, rotate, copy
+ * cur -> prev.  Only cur is observable (download), so two buffers suffice. */
+void oracle_heat_run_f32(const float* t0, float* buf0, float* buf1, float* out, size_t nx,
+                         size_t ny, size_t nz, float dx, float dy, float dz, float dt_eff,
+                         float alpha, int n_steps) {
+    const size_t n = nx * ny * nz;
+    float* a = buf0;
+    float* b = buf1;
+    memcpy(a, t0, n * sizeof(float));
+    memset(b, 0, n * sizeof(float));
+    for (int s = 0; s < n_steps; ++s) {
+        oracle_heat_step_f32(a, b, nx, ny, nz, dx, dy, dz, dt_eff, alpha);
+        float* t = a;
+        a = b;
+        b = t;
+    }
+    memcpy(out, a, n * sizeof(float));
+}
+
+/* ---- Seeded synthetic inputs (no reference counterpart). ---- */
+
+/* SplitMix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom
+ * number generators", OOPSLA 2014). */
+uint64_t oracle_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+/* U[0,1) with 24 random bits, keyed by (seed, stream, idx). */
+float oracle_u01(uint64_t seed, uint64_t stream, uint64_t idx) {
+    const uint64_t k = oracle_splitmix64(seed ^ oracle_splitmix64(stream)) + idx;
+    return (float)(oracle_splitmix64(k) >> 40) * (1.0f / 16777216.0f);
+}
+
+void oracle_layered_noise_vel2_f32(float* vel2, size_t nx, size_t ny, size_t nz, int n_layers,
+                                   float vmin, float vmax, float noise, uint64_t seed) {
+    for (size_t iz = 0; iz < nz; ++iz) {
+        size_t layer = iz * (size_t)n_layers / nz;
+        const float vl = vmin + (vmax - vmin) * oracle_u01(seed, 1, layer);
+        for (size_t iy = 0; iy < ny; ++iy)
+            for (size_t ix = 0; ix < nx; ++ix) {
+                const size_t i = iz * nx * ny + iy * nx + ix;
+                const float f = 1.0f + noise * (2.0f * oracle_u01(seed, 2, i) - 1.0f);
+                const float v = vl * f;
+                vel2[i] = v * v;
+            }
+    }
+}
+
+void oracle_heat_noise_cube_f32(float* t, size_t nx, size_t ny, size_t nz, size_t side,
+                                float hot, uint64_t seed) {
+    const size_t x0 = nx / 2 - side / 2, y0 = ny / 2 - side / 2, z0 = nz / 2 - side / 2;
+    memset(t, 0, nx * ny * nz * sizeof(float));
+    for (size_t iz = R; iz < nz - R; ++iz)
+        for (size_t iy = R; iy < ny - R; ++iy)
+            for (size_t ix = R; ix < nx - R; ++ix) {
+                const size_t i = iz * nx * ny + iy * nx + ix;
+                const int in_cube = ix >= x0 && ix < x0 + side && iy >= y0 && iy < y0 + side &&
+                                    iz >= z0 && iz < z0 + side;
+                t[i] = in_cube ? hot : oracle_u01(seed, 3, i);
+            }
+}
+
+double oracle_rel_l2_interior_f32(const float* a, const float* b, size_t nx, size_t ny,
+                                  size_t nz) {
+    double num = 0.0, den = 0.0;
+    for (size_t iz = R; iz < nz - R; ++iz)
+        for (size_t iy = R; iy < ny - R; ++iy)
+            for (size_t ix = R; ix < nx - R; ++ix) {
+                const size_t i = iz * nx * ny + iy * nx + ix;
+                const double d = (double)a[i] - (double)b[i];
+                num += d * d;
+                den += (double)b[i] * (double)b[i];
+            }
+    return den > 0.0 ? sqrt(num / den) : sqrt(num);
+}
+
+double oracle_rel_l2_interior_f64f32(const float* a, const double* b, size_t nx, size_t ny,
+                                     size_t nz) {
+    double num = 0.0, den = 0.0;
+    for (size_t iz = R; iz < nz - R; ++iz)
+        for (size_t iy = R; iy < ny - R; ++iy)
+            for (size_t ix = R; ix < nx - R; ++ix) {
+                const size_t i = iz * nx * ny + iy * nx + ix;
+                const double d = (double)a[i] - b[i];
+                num += d * d;
+                den += b[i] * b[i];
+            }
+    return den > 0.0 ? sqrt(num / den) : sqrt(num);
+}

@@ -1,0 +1,316 @@
+"""Python mirror of the reference's operator and solver API over the C-ABI.
+
+* ``launch_naive / launch_tiled / launch / upload_fd_coefficients`` -- the
+  launcher API (reference include/stencil/kernels.cuh:33-83) on torch device
+  tensors; asynchronous on the current torch stream unless ``stream`` is given.
+* ``heat_step`` -- the dedicated heat kernel (replaces the heat step of
+  src/solver/heat_diffusion.cpp:103-122).
+* ``Config`` / ``make_solver`` / ``Solver`` -- the solver API (reference
+  include/stencil/config.hpp:34-88, include/stencil/solver.hpp:13-24).
+
+Fields are torch tensors of shape (nz, ny, nx), C-contiguous, so that the
+flat index is z*nx*ny + y*nx + x as in the reference layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import StencilError, check, lib, stencil_config, stencil_grid, stencil_perf
+
+VARIANTS = {"naive": 0, "tiled": 1, "multigpu": 2}
+RADIUS = 4
+
+
+def _variant(v) -> int:
+    if isinstance(v, str):
+        if v not in VARIANTS:
+            raise ValueError(f"unknown kernel variant {v!r}")
+        return VARIANTS[v]
+    return int(v)
+
+
+@dataclass
+class GridDims:
+    """Reference GridDims (include/stencil/types.hpp:115-140)."""
+
+    nx: int
+    ny: int
+    nz: int
+    dx: float = 1.0
+    dy: float = 1.0
+    dz: float = 1.0
+    dt: float = 1.0
+
+    @classmethod
+    def cube(cls, n: int, h: float = 1.0, dt: float = 1.0) -> "GridDims":
+        return cls(n, n, n, h, h, h, dt)
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def size(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def interior_size(self) -> int:
+        return max(self.nx - 2 * RADIUS, 0) * max(self.ny - 2 * RADIUS, 0) * max(self.nz - 2 * RADIUS, 0)
+
+    def to_c(self) -> stencil_grid:
+        return stencil_grid(self.nx, self.ny, self.nz, float(self.dx), float(self.dy), float(self.dz),
+                            float(self.dt))
+
+
+def _dev_ptr(t, grid: GridDims, dtype, name: str) -> int:
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a torch tensor on a HIP device")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous() or t.numel() != grid.size():
+        raise ValueError(f"{name} must be contiguous with nx*ny*nz = {grid.size()} elements")
+    return t.data_ptr()
+
+
+def _stream(stream) -> int:
+    import torch
+
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def upload_fd_coefficients() -> None:
+    check(lib().stencil_upload_fd_coefficients(), "upload_fd_coefficients")
+
+
+def _wave(fn32, fn64, p_cur, p_prev, vel2, p_next, grid: GridDims, stream, what, extra=()):
+    import torch
+
+    dtype = p_cur.dtype
+    if dtype not in (torch.float32, torch.float64):
+        raise TypeError("fields must be float32 or float64")
+    fn = fn32 if dtype == torch.float32 else fn64
+    if fn is None:
+        raise TypeError(f"{what} has no {dtype} form")
+    ptrs = [_dev_ptr(t, grid, dtype, n) for t, n in
+            ((p_cur, "p_cur"), (p_prev, "p_prev"), (vel2, "vel2"), (p_next, "p_next"))]
+    g = grid.to_c()
+    check(fn(*ptrs, C.byref(g), *extra, _stream(stream)), what)
+
+
+def launch_naive(p_cur, p_prev, vel2, p_next, grid: GridDims, stream=None) -> None:
+    """One acoustic step, coalesced kernel, reference association (kernels.cuh:42)."""
+    L = lib()
+    _wave(L.stencil_launch_naive_f32, L.stencil_launch_naive_f64, p_cur, p_prev, vel2, p_next, grid,
+          stream, "launch_naive")
+
+
+def launch_tiled(p_cur, p_prev, vel2, p_next, grid: GridDims, stream=None) -> None:
+    """One acoustic step, LDS + z-register-queue kernel (kernels.cuh:54)."""
+    L = lib()
+    _wave(L.stencil_launch_tiled_f32, L.stencil_launch_tiled_f64, p_cur, p_prev, vel2, p_next, grid,
+          stream, "launch_tiled")
+
+
+def launch(variant, p_cur, p_prev, vel2, p_next, grid: GridDims, stream=None) -> None:
+    """Dispatch by KernelVariant (kernels.cuh:68-83)."""
+    v = _variant(variant)
+    if v == 0:
+        launch_naive(p_cur, p_prev, vel2, p_next, grid, stream)
+    elif v in (1, 2):
+        launch_tiled(p_cur, p_prev, vel2, p_next, grid, stream)
+    else:
+        raise ValueError(f"unknown kernel variant {variant!r}")
+
+
+def launch_tiled_ex(p_cur, p_prev, vel2, p_next, grid: GridDims, exact=False, ty=0, n_chunks=0,
+                    xcd_remap=-1, stream=None) -> None:
+    """The tiled step with an explicit launch shape (no reference counterpart)."""
+    L = lib()
+    _wave(L.stencil_launch_tiled_ex_f32, None, p_cur, p_prev, vel2, p_next, grid, stream,
+          "launch_tiled_ex", extra=(int(bool(exact)), int(ty), int(n_chunks), int(xcd_remap)))
+
+
+def heat_step(t, t_next, alpha: float, grid: GridDims, variant="tiled", stream=None, exact=None,
+              ty=0, n_chunks=0, xcd_remap=-1) -> None:
+    """t_next = t + (dt^2 alpha) Lap(t); grid.dt is the heat dt_eff."""
+    import torch
+
+    L = lib()
+    g = grid.to_c()
+    if t.dtype == torch.float64:
+        ptrs = [_dev_ptr(x, grid, torch.float64, n) for x, n in ((t, "t"), (t_next, "t_next"))]
+        check(L.stencil_heat_step_f64(*ptrs, float(alpha), C.byref(g), _variant(variant), _stream(stream)),
+              "heat_step")
+        return
+    ptrs = [_dev_ptr(x, grid, torch.float32, n) for x, n in ((t, "t"), (t_next, "t_next"))]
+    if exact is None:
+        check(L.stencil_heat_step_f32(*ptrs, float(alpha), C.byref(g), _variant(variant), _stream(stream)),
+              "heat_step")
+    else:
+        check(L.stencil_heat_step_ex_f32(*ptrs, float(alpha), C.byref(g), int(bool(exact)), int(ty),
+                                         int(n_chunks), int(xcd_remap), _stream(stream)), "heat_step_ex")
+
+
+def wave_bytes_per_step(grid: GridDims, bytes_per_value: int = 4) -> float:
+    g = grid.to_c()
+    return float(lib().stencil_wave_bytes_per_step(C.byref(g), bytes_per_value))
+
+
+def heat_bytes_per_step(grid: GridDims, bytes_per_value: int = 4) -> float:
+    g = grid.to_c()
+    return float(lib().stencil_heat_bytes_per_step(C.byref(g), bytes_per_value))
+
+
+# ── Seeded host inputs ──────────────────────────────────────────────────────
+def host_gaussian(shape, cx=0.5, cy=0.5, cz=0.5, sigma=5.0) -> np.ndarray:
+    """init_gaussian (reference include/stencil/grid.hpp:57-78) into a new array."""
+    nz, ny, nx = shape
+    a = np.zeros(shape, dtype=np.float32)
+    check(lib().stencil_host_init_gaussian_f32(a.ctypes.data, nx, ny, nz, cx, cy, cz, sigma), "host_gaussian")
+    return a
+
+
+def host_layered_noise_vel2(shape, n_layers=16, vmin=1500.0, vmax=4000.0, noise=0.05, seed=2) -> np.ndarray:
+    nz, ny, nx = shape
+    a = np.empty(shape, dtype=np.float32)
+    check(lib().stencil_host_layered_noise_vel2_f32(a.ctypes.data, nx, ny, nz, n_layers, vmin, vmax, noise,
+                                                    seed), "host_layered_noise_vel2")
+    return a
+
+
+def host_noise_cube(shape, side=64, hot=10.0, seed=2) -> np.ndarray:
+    nz, ny, nx = shape
+    a = np.empty(shape, dtype=np.float32)
+    check(lib().stencil_host_noise_cube_f32(a.ctypes.data, nx, ny, nz, side, hot, seed), "host_noise_cube")
+    return a
+
+
+# ── Solver API ──────────────────────────────────────────────────────────────
+@dataclass
+class Config:
+    """Reference Config (include/stencil/config.hpp:34-88) plus the additions
+    of include/stencil/config.hpp (dt, velocity model, initial condition)."""
+
+    nx: int = 256
+    ny: int = 256
+    nz: int = 256
+    dx: float = 10.0
+    dy: float = 10.0
+    dz: float = 10.0
+    dt: float = 0.0
+    steps: int = 100
+    warmup_steps: int = 10
+    velocity: float = 1500.0
+    diffusivity: float = float(np.float32(1.0e-4))
+    pulse_cx: float = 0.5
+    pulse_cy: float = 0.5
+    pulse_cz: float = 0.5
+    pulse_sigma: float = 5.0
+    physics: str = "wave"
+    variant: str = "naive"
+    device: int = 0
+    vmodel: str = "constant"
+    n_layers: int = 16
+    vmin: float = 1500.0
+    vmax: float = 4000.0
+    vnoise: float = 0.05
+    ic: str = "gaussian"
+    cube_side: int = 64
+    cube_value: float = 10.0
+    seed: int = 2
+    fp64: bool = False
+
+    def to_c(self) -> stencil_config:
+        c = stencil_config()
+        lib().stencil_config_default(C.byref(c))
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "physics":
+                v = {"wave": 0, "heat": 1}[v]
+            elif f.name == "variant":
+                v = _variant(v)
+            elif f.name == "vmodel":
+                v = {"constant": 0, "layered-noise": 1}[v]
+            elif f.name == "ic":
+                v = {"gaussian": 0, "noise-cube": 1}[v]
+            elif f.name == "fp64":
+                v = int(bool(v))
+            setattr(c, f.name, v)
+        return c
+
+
+@dataclass
+class PerfResult:
+    """Reference PerfResult (include/stencil/timer.hpp:88-96)."""
+
+    label: str = ""
+    time_ms: float = 0.0
+    bandwidth_gbs: float = 0.0
+    gflops: float = 0.0
+    model_bandwidth_gbs: float = 0.0
+    grid_size: int = 0
+
+
+class Solver:
+    """ISolver over a C-ABI handle (reference include/stencil/solver.hpp:13-22)."""
+
+    def __init__(self, cfg: Config):
+        self._h = C.c_void_p()
+        c = cfg.to_c()
+        check(lib().stencil_solver_create(C.byref(c), C.byref(self._h)), "make_solver")
+        self.cfg = cfg
+
+    def close(self):
+        if self._h:
+            lib().stencil_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self) -> None:
+        check(lib().stencil_solver_step(self._h), "step")
+
+    def run(self, n_steps: int, warmup: int = 0) -> PerfResult:
+        p = stencil_perf()
+        check(lib().stencil_solver_run(self._h, n_steps, warmup, C.byref(p)), "run")
+        return PerfResult(p.label.decode(), p.time_ms, p.bandwidth_gbs, p.gflops, p.model_bandwidth_gbs,
+                          p.grid_size)
+
+    def dims(self) -> GridDims:
+        g = stencil_grid()
+        check(lib().stencil_solver_dims(self._h, C.byref(g)), "dims")
+        return GridDims(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.dt)
+
+    def name(self) -> str:
+        buf = C.create_string_buffer(64)
+        check(lib().stencil_solver_name(self._h, buf, 64), "name")
+        return buf.value.decode()
+
+    def download(self) -> np.ndarray:
+        d = self.dims()
+        out = np.empty(d.shape, dtype=np.float64 if self.cfg.fp64 else np.float32)
+        check(lib().stencil_solver_download(self._h, out.ctypes.data, out.nbytes), "download")
+        return out
+
+
+def make_solver(cfg: Config) -> Solver:
+    """Reference make_solver (src/solver/solver_base.cpp:119-133)."""
+    return Solver(cfg)
+
+
+__all__ = ["GridDims", "Config", "PerfResult", "Solver", "StencilError", "make_solver", "upload_fd_coefficients",
+           "launch_naive", "launch_tiled", "launch", "launch_tiled_ex", "heat_step", "wave_bytes_per_step",
+           "heat_bytes_per_step", "host_gaussian", "host_layered_noise_vel2", "host_noise_cube", "VARIANTS"]

@@ -1,0 +1,65 @@
+// Host-side launchers of the gfx950 stencil kernels (internal to the library;
+// the public surface is include/stencil_capi.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace stencil::hipk {
+
+// Total cells per axis, halo included.
+struct Dims {
+    int64_t nx = 0, ny = 0, nz = 0;
+};
+
+// The per-launch scalars the reference launchers form on the host
+// (src/kernels/stencil_naive.cu:96-99): inv_h2 = 1/(h*h), dt2 = dt*dt in T.
+template <class T>
+struct Scalars {
+    T inv_dx2, inv_dy2, inv_dz2, dt2;
+};
+
+template <class T>
+Scalars<T> make_scalars(T dx, T dy, T dz, T dt) {
+    Scalars<T> s;
+    s.inv_dx2 = T(1) / (dx * dx);
+    s.inv_dy2 = T(1) / (dy * dy);
+    s.inv_dz2 = T(1) / (dz * dz);
+    s.dt2 = dt * dt;
+    return s;
+}
+
+// Launch shape of the LDS + z-register-queue kernel.  Zero fields = choose
+// automatically (see zq_resolve).
+struct ZqShape {
+    int ty = 0;        // tile rows: 16, 24 or 32 (tile is 64 x ty; 16*ty threads)
+    int n_chunks = 0;  // z-chunks per column (each re-reads 8 halo planes)
+    int xcd_remap = -1;  // 1 = XCD-contiguous tile order, 0 = off, -1 = auto
+};
+
+// Fills unset fields and reports whether the kernel can take these dims and
+// pointers (nx % 4 == 0, 16-byte aligned bases).  Returns false if not.
+bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
+ZqShape zq_resolve(const Dims& d, ZqShape s);
+
+// The acoustic step.  `exact` = the reference association (bit-identical to
+// its CPU path); otherwise FMA-fused weights.
+hipError_t wave_naive_f32(const float* cur, const float* prev, const float* vel2, float* next,
+                          const Dims& d, const Scalars<float>& s, hipStream_t st);
+hipError_t wave_naive_f64(const double* cur, const double* prev, const double* vel2,
+                          double* next, const Dims& d, const Scalars<double>& s, hipStream_t st);
+hipError_t wave_zq_f32(const float* cur, const float* prev, const float* vel2, float* next,
+                       const Dims& d, const Scalars<float>& s, bool exact, ZqShape shape,
+                       hipStream_t st);
+
+// The heat step t_next = t + coef * Lap(t), coef = dt2 * alpha formed in float
+// (the reference's dt2 * vel2[c] with vel2 == alpha).
+hipError_t heat_naive_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,
+                          float coef, hipStream_t st);
+hipError_t heat_naive_f64(const double* t, double* tn, const Dims& d, const Scalars<double>& s,
+                          double coef, hipStream_t st);
+hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,
+                       float coef, bool exact, ZqShape shape, hipStream_t st);
+
+}  // namespace stencil::hipk

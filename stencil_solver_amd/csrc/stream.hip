@@ -1,0 +1,87 @@
+// HBM calibration kernels (the STREAM-style copy / read / write of the
+// reference's tests/perf/bench_stream.cu:53-101), dwordx4 per lane, grid-stride.
+// They give the achievable-bandwidth line next to the stencil's roofline
+// fraction and the known-size copy that scales rocprof's FETCH/WRITE counters.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "stencil_capi.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void copy_kernel(const float4* __restrict__ src,
+                                                        float4* __restrict__ dst, size_t n4) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kThreads) void read_kernel(const float4* __restrict__ src,
+                                                        float* __restrict__ sink, size_t n4) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    float acc = 0.f;
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n4; i += stride) {
+        const float4 v = src[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    // One store per workgroup keeps the loads live without measurable writes.
+    __shared__ float red[kThreads];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sink[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void write_kernel(float4* __restrict__ dst, float value,
+                                                         size_t n4) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    const float4 v = make_float4(value, value, value, value);
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n4; i += stride) dst[i] = v;
+}
+
+unsigned grid_for(size_t n4) {
+    // 8 workgroups per CU on 256 CUs saturates HBM; never more than the work.
+    const size_t want = 2048;
+    const size_t need = (n4 + kThreads - 1) / kThreads;
+    return static_cast<unsigned>(need < want ? (need ? need : 1) : want);
+}
+
+}  // namespace
+
+extern "C" {
+
+int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* stream) {
+    if (n == 0 || n % 4 != 0) return STENCIL_ERR_INVALID;
+    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16 != 0)
+        return STENCIL_ERR_INVALID;
+    const size_t n4 = n / 4;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const unsigned g = grid_for(n4);
+    switch (op) {
+        case 0:  // copy: src -> dst
+            if (!dst || !src || dst == src) return STENCIL_ERR_INVALID;
+            hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(kThreads), 0, st,
+                               reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
+            break;
+        case 1:  // read: src, one partial sum per workgroup into dst[0..grid)
+            if (!dst || !src) return STENCIL_ERR_INVALID;
+            hipLaunchKernelGGL(read_kernel, dim3(g), dim3(kThreads), 0, st,
+                               reinterpret_cast<const float4*>(src), dst, n4);
+            break;
+        case 2:  // write: dst := 1.0
+            if (!dst) return STENCIL_ERR_INVALID;
+            hipLaunchKernelGGL(write_kernel, dim3(g), dim3(kThreads), 0, st,
+                               reinterpret_cast<float4*>(dst), 1.0f, n4);
+            break;
+        default:
+            return STENCIL_ERR_INVALID;
+    }
+    return hipGetLastError() == hipSuccess ? STENCIL_OK : STENCIL_ERR_HIP;
+}
+
+}  // extern "C"

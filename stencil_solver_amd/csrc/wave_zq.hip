@@ -1,0 +1,330 @@
+// LDS-staged z-pencil register-queue stencil kernel for gfx950 (the "tiled"
+// variant; replaces the reference's shared-memory tile + register pencil,
+// src/kernels/stencil_tiled.cu:37-145).
+//
+// Work decomposition.  A workgroup owns a 64 x TY tile of (x, y) columns and
+// marches through one z-chunk of planes.  Each thread owns 4 consecutive x
+// (one 16-byte float4) of one row, so a wave64 covers 4 rows of 256 bytes and
+// every global access is a dwordx4.
+//   * z neighbours: a 9-deep register queue of float4 per thread (planes
+//     z-4..z+4), refilled with one load per plane; the z loop is unrolled by 9
+//     so queue slots are renamed, never moved.
+//   * x / y neighbours: plane z is staged in LDS as (TY+8) rows x 18 float4
+//     (64 + 2x4 floats), written from the queue centre plus one halo float4 per
+//     loader thread (top/bottom 4 rows, left/right 4 columns; corners are never
+//     read by the star stencil).  LDS is double-buffered, so one barrier per
+//     plane suffices.
+//   * streamed fields (p_prev, vel2, p_next) are touched once, non-temporally,
+//     and the halo / stream loads for plane z+1 are issued before plane z is
+//     computed (gfx950 barriers do not drain outstanding global loads).
+// HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
+// write p_next once = 16 B (halo re-reads are served by L2, kept XCD-local by
+// the tile order).
+#include "device_common.hpp"
+#include "launchers.hpp"
+
+namespace stencil::hipk {
+namespace {
+
+using dev::comp;
+using dev::Fd;
+using dev::kR;
+
+enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3 };
+
+struct ZqArgs {
+    const float* cur;
+    const float* prev;
+    const float* vel2;
+    float* next;
+    int nx, ny, nz;
+    int tiles_x, tiles_y, chunk;
+    int xcd_remap;
+    Scalars<float> s;
+    float coef;                      // heat: dt2 * alpha
+    float w0, wx[4], wy[4], wz[4];   // fused weights (heat: pre-multiplied by coef)
+};
+
+constexpr int kCols = 18;  // 16 own float4 + left/right halo float4
+
+template <int TY, int MODE>
+__global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
+    constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
+    constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact);
+    constexpr int kRows = TY + 2 * kR;
+    constexpr int kHalo = 8 * 16 + 2 * TY;  // halo float4 per plane
+    static_assert(kHalo <= 16 * TY, "one halo float4 per thread at most");
+    __shared__ float4 lds[2][kRows][kCols];
+
+    const unsigned logical = dev::xcd_remap(blockIdx.x, gridDim.x, a.xcd_remap != 0);
+    const int tile_x = static_cast<int>(logical % static_cast<unsigned>(a.tiles_x));
+    const unsigned rest = logical / static_cast<unsigned>(a.tiles_x);
+    const int tile_y = static_cast<int>(rest % static_cast<unsigned>(a.tiles_y));
+    const int chunk_id = static_cast<int>(rest / static_cast<unsigned>(a.tiles_y));
+
+    const int tx = static_cast<int>(threadIdx.x);
+    const int ty = static_cast<int>(threadIdx.y);
+    const int tid = ty * 16 + tx;
+    const int x0 = kR + tile_x * 64;
+    const int y0 = kR + tile_y * TY;
+    const int x = x0 + 4 * tx;
+    const int y = y0 + ty;
+    const int z_begin = kR + chunk_id * a.chunk;
+    const int z_end = min(z_begin + a.chunk, a.nz - kR);
+    if (z_begin >= z_end) return;  // uniform per workgroup
+
+    const size_t plane = static_cast<size_t>(a.nx) * static_cast<size_t>(a.ny);
+    const bool own_in = (x < a.nx) && (y < a.ny);
+    const bool own_out = (x < a.nx - kR) && (y < a.ny - kR);
+    const unsigned own_off = static_cast<unsigned>(y) * static_cast<unsigned>(a.nx) +
+                             static_cast<unsigned>(x);
+
+    // This thread's halo float4 (if any): fixed (row, col) of the LDS tile.
+    int h_row = 0, h_col = 0;
+    bool h_in = false;
+    unsigned h_off = 0;
+    if (tid < kHalo) {
+        if (tid < 64) {
+            h_row = tid >> 4;
+            h_col = 1 + (tid & 15);
+        } else if (tid < 128) {
+            h_row = TY + kR + ((tid - 64) >> 4);
+            h_col = 1 + (tid & 15);
+        } else if (tid < 128 + TY) {
+            h_row = kR + (tid - 128);
+            h_col = 0;
+        } else {
+            h_row = kR + (tid - 128 - TY);
+            h_col = kCols - 1;
+        }
+        const int hy = y0 - kR + h_row;
+        const int hx = x0 - 4 + 4 * h_col;
+        h_in = (hy < a.ny) && (hx < a.nx);
+        h_off = static_cast<unsigned>(hy) * static_cast<unsigned>(a.nx) +
+                static_cast<unsigned>(hx);
+    }
+
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    // Predicated loads; the zero is built in registers so the compiler cannot
+    // turn the predicate into a select between a global and a private address.
+    const float* const cur_base = a.cur;
+    auto cur_at = [cur_base, plane](int z, unsigned off, bool ok) -> float4 {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) v = *reinterpret_cast<const float4*>(cur_base + static_cast<size_t>(z) * plane + off);
+        return v;
+    };
+    auto stream_at = [plane, own_off, own_out](const float* f, int z) -> float4 {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (own_out)
+            v = dev::load_stream(
+                reinterpret_cast<const float4*>(f + static_cast<size_t>(z) * plane + own_off));
+        return v;
+    };
+
+    // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
+    float4 q[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = cur_at(z_begin - kR + k, own_off, own_in);
+    float4 h_cur = cur_at(z_begin, h_off, h_in);
+    float4 pv_cur = zero, vv_cur = zero;
+    if constexpr (!kHeat) {
+        pv_cur = stream_at(a.prev, z_begin);
+        vv_cur = stream_at(a.vel2, z_begin);
+    }
+
+    for (int zb = z_begin; zb < z_end; zb += 9) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int zz = zb + j;
+            if (zz < z_end) {  // uniform per workgroup; a constant trip count keeps the unroll
+            // Refill the queue (plane zz+4) and prefetch plane zz+1's halo and
+            // streams; all in bounds since zz+4 <= nz-1.
+            q[(j + 8) % 9] = cur_at(zz + kR, own_off, own_in);
+            const float4 h_nxt = cur_at(zz + 1, h_off, h_in);
+            float4 pv_nxt = zero, vv_nxt = zero;
+            if constexpr (!kHeat) {
+                pv_nxt = stream_at(a.prev, zz + 1);
+                vv_nxt = stream_at(a.vel2, zz + 1);
+            }
+
+            const int b = (zz - z_begin) & 1;
+            const float4 ctr4 = q[(j + 4) % 9];
+            lds[b][ty + kR][tx + 1] = ctr4;
+            if (tid < kHalo) lds[b][h_row][h_col] = h_cur;
+            __syncthreads();
+
+            if (own_out) {
+                const float4 xl = lds[b][ty + kR][tx];
+                const float4 xr = lds[b][ty + kR][tx + 2];
+                const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
+                                     ctr4.z, ctr4.w, xr.x,   xr.y,   xr.z,   xr.w};
+                float4 ym[kR], yp[kR];
+#pragma unroll
+                for (int r = 1; r <= kR; ++r) {
+                    ym[r - 1] = lds[b][ty + kR - r][tx + 1];
+                    yp[r - 1] = lds[b][ty + kR + r][tx + 1];
+                }
+                float4 out;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float p = w[4 + i];
+                    float res;
+                    if constexpr (kExact) {
+                        // src/kernels/stencil_naive.cu:75-88, op for op.
+                        float lx = Fd<float>::c0 * p * a.s.inv_dx2;
+                        float ly = Fd<float>::c0 * p * a.s.inv_dy2;
+                        float lz = Fd<float>::c0 * p * a.s.inv_dz2;
+#pragma unroll
+                        for (int r = 1; r <= kR; ++r) {
+                            lx += Fd<float>::c(r) * (w[4 + i + r] + w[4 + i - r]) * a.s.inv_dx2;
+                            ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) *
+                                  a.s.inv_dy2;
+                            lz += Fd<float>::c(r) *
+                                  (comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i)) *
+                                  a.s.inv_dz2;
+                        }
+                        if constexpr (kHeat) {
+                            res = p + a.coef * (lx + ly + lz);
+                        } else {
+                            res = 2.f * p - comp(pv_cur, i) +
+                                  a.s.dt2 * comp(vv_cur, i) * (lx + ly + lz);
+                        }
+                    } else {
+                        // Combined weights w_r = c_r / h^2, one accumulator.
+                        float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
+#pragma unroll
+                        for (int r = 1; r <= kR; ++r) {
+                            acc = __builtin_fmaf(a.wx[r - 1], w[4 + i + r] + w[4 + i - r], acc);
+                            acc = __builtin_fmaf(a.wy[r - 1],
+                                                 comp(yp[r - 1], i) + comp(ym[r - 1], i), acc);
+                            acc = __builtin_fmaf(
+                                a.wz[r - 1],
+                                comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i), acc);
+                        }
+                        if constexpr (kHeat) {
+                            res = acc;
+                        } else {
+                            const float base = __builtin_fmaf(2.f, p, -comp(pv_cur, i));
+                            res = __builtin_fmaf(a.s.dt2 * comp(vv_cur, i), acc, base);
+                        }
+                    }
+                    dev::set_comp(out, i, res);
+                }
+                float* nb = a.next + static_cast<size_t>(zz) * plane;
+                dev::store_stream(reinterpret_cast<float4*>(nb + own_off), out);
+            }
+            h_cur = h_nxt;
+            pv_cur = pv_nxt;
+            vv_cur = vv_nxt;
+            }
+        }
+    }
+}
+
+using KernelFn = void (*)(ZqArgs);
+
+template <int MODE>
+KernelFn pick_kernel(int ty) {
+    switch (ty) {
+        case 16: return zq_kernel<16, MODE>;
+        case 24: return zq_kernel<24, MODE>;
+        case 32: return zq_kernel<32, MODE>;
+        default: return nullptr;
+    }
+}
+
+KernelFn pick(int ty, int mode) {
+    switch (mode) {
+        case kWaveFused: return pick_kernel<kWaveFused>(ty);
+        case kWaveExact: return pick_kernel<kWaveExact>(ty);
+        case kHeatFused: return pick_kernel<kHeatFused>(ty);
+        default: return pick_kernel<kHeatExact>(ty);
+    }
+}
+
+// Fused weights: c_r / h^2 formed in double from the exact rationals and the
+// launcher's float 1/h^2, rounded once.
+void fill_weights(ZqArgs& a, double scale) {
+    const double c[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+    const double ix = a.s.inv_dx2, iy = a.s.inv_dy2, iz = a.s.inv_dz2;
+    a.w0 = static_cast<float>(scale * c[0] * (ix + iy + iz));
+    for (int r = 1; r <= 4; ++r) {
+        a.wx[r - 1] = static_cast<float>(scale * c[r] * ix);
+        a.wy[r - 1] = static_cast<float>(scale * c[r] * iy);
+        a.wz[r - 1] = static_cast<float>(scale * c[r] * iz);
+    }
+}
+
+hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st) {
+    if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
+    shape = zq_resolve(d, shape);
+    KernelFn fn = pick(shape.ty, mode);
+    if (!fn) return hipErrorInvalidValue;
+    const int64_t inz = d.nz - 2 * kR;
+    a.nx = static_cast<int>(d.nx);
+    a.ny = static_cast<int>(d.ny);
+    a.nz = static_cast<int>(d.nz);
+    a.tiles_x = static_cast<int>((d.nx - 2 * kR + 63) / 64);
+    a.tiles_y = static_cast<int>((d.ny - 2 * kR + shape.ty - 1) / shape.ty);
+    a.chunk = static_cast<int>((inz + shape.n_chunks - 1) / shape.n_chunks);
+    const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
+    const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
+    a.xcd_remap = (shape.xcd_remap != 0 && nwg % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg)), dim3(16, shape.ty, 1), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs) {
+    if (d.nx % 4 != 0) return false;
+    // 32-bit in-plane offsets.
+    if (d.nx * d.ny >= (int64_t(1) << 32)) return false;
+    for (int i = 0; i < n_ptrs; ++i)
+        if (reinterpret_cast<uintptr_t>(ptrs[i]) % 16 != 0) return false;
+    return true;
+}
+
+ZqShape zq_resolve(const Dims& d, ZqShape s) {
+    if (s.ty != 16 && s.ty != 24 && s.ty != 32) s.ty = 16;
+    const int64_t inz = d.nz - 2 * kR;
+    if (s.n_chunks <= 0) {
+        // Aim for >= 1024 workgroups (4 per CU on 256 CUs) with chunks of at
+        // least 32 planes, rounded so the total is a multiple of 8 (XCDs).
+        const int64_t tiles = ((d.nx - 2 * kR + 63) / 64) * ((d.ny - 2 * kR + s.ty - 1) / s.ty);
+        int64_t want = (1024 + tiles - 1) / tiles;
+        const int64_t max_chunks = inz >= 32 ? inz / 32 : 1;
+        if (want > max_chunks) want = max_chunks;
+        if (want < 1) want = 1;
+        s.n_chunks = static_cast<int>(want);
+    }
+    if (s.n_chunks > inz) s.n_chunks = static_cast<int>(inz);
+    if (s.n_chunks < 1) s.n_chunks = 1;
+    return s;
+}
+
+hipError_t wave_zq_f32(const float* cur, const float* prev, const float* vel2, float* next,
+                       const Dims& d, const Scalars<float>& s, bool exact, ZqShape shape,
+                       hipStream_t st) {
+    ZqArgs a{};
+    a.cur = cur;
+    a.prev = prev;
+    a.vel2 = vel2;
+    a.next = next;
+    a.s = s;
+    fill_weights(a, 1.0);
+    return launch_zq(a, d, shape, exact ? kWaveExact : kWaveFused, st);
+}
+
+hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,
+                       float coef, bool exact, ZqShape shape, hipStream_t st) {
+    ZqArgs a{};
+    a.cur = t;
+    a.next = tn;
+    a.s = s;
+    a.coef = coef;
+    fill_weights(a, static_cast<double>(coef));
+    return launch_zq(a, d, shape, exact ? kHeatExact : kHeatFused, st);
+}
+
+}  // namespace stencil::hipk

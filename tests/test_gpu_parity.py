@@ -1,0 +1,321 @@
+"""Parity of the gfx950 kernels with the oracle and the reference's golden vectors.
+
+Bars (tolerances written in each test):
+* naive kernel, FP32 and FP64, and the tiled kernel in its exact-association
+  mode: bit-identical to the oracle / to the reference CPU path built without
+  FMA contraction.
+* tiled kernel in its default FMA-fused form: relative L2 <= 1e-5 against the
+  oracle (the north-star FP32 tolerance), and against the reference's own
+  Release build.
+* at the full 512^3 / 1024^3 sizes (too large for the oracle): size-independent
+  properties -- exact-mode tiled == naive bit for bit, fused within 1e-5 of
+  naive, halos untouched, mirror symmetry of a centred pulse.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+R = 4
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def to_dev(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def ring_run(ss, variant, p0, vel2, grid, n_steps, dev, **kw):
+    """n_steps over a 3-buffer ring as WavePropSolver::step_impl does; returns cur."""
+    import torch
+
+    bufs = [to_dev(p0, dev), to_dev(p0, dev), torch.zeros(p0.shape, dtype=to_dev(p0, dev).dtype, device=dev)]
+    v = to_dev(vel2, dev)
+    i = 0
+    for _ in range(n_steps):
+        if variant == "tiled_ex":
+            ss.launch_tiled_ex(bufs[(i + 1) % 3], bufs[i], v, bufs[(i + 2) % 3], grid, **kw)
+        else:
+            ss.launch(variant, bufs[(i + 1) % 3], bufs[i], v, bufs[(i + 2) % 3], grid)
+        i = (i + 1) % 3
+    torch.cuda.synchronize()
+    return bufs[(i + 1) % 3].cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def ss(dev):
+    import stencil_solver_amd
+
+    return stencil_solver_amd
+
+
+# ── golden vectors of the reference CPU path ──────────────────────────────────
+def test_naive_step_matches_reference_golden(ss, dev):
+    gd = load_golden("kernel_24.npz")
+    g = gd["init"]
+    vel2 = np.full_like(g, np.float32(1500.0) * np.float32(1500.0))
+    grid = ss.GridDims.cube(24, np.float32(10.0), np.float32(1e-3))
+    import torch
+
+    nxt = torch.zeros(g.shape, device=dev)
+    ss.launch_naive(to_dev(g, dev), to_dev(g, dev), to_dev(vel2, dev), nxt, grid)
+    out = nxt.cpu().numpy()
+    assert np.array_equal(out, gd["next_nofma"])  # bit-exact
+    t = torch.zeros(g.shape, device=dev)
+    ss.launch_tiled(to_dev(g, dev), to_dev(g, dev), to_dev(vel2, dev), t, grid)
+    tt = t.cpu().numpy()
+    rel = np.linalg.norm(tt - gd["next_nofma"]) / np.linalg.norm(gd["next_nofma"])
+    assert rel <= 1e-5 and np.max(np.abs(tt - out)) < 1e-5
+
+
+@pytest.mark.parametrize("variant", ["naive", "tiled", "multigpu"])
+@pytest.mark.parametrize("steps", [1, 20])
+def test_wave_solver_matches_reference_golden(ss, dev, variant, steps):
+    gd = load_golden("solver_wave_32.npz")
+    s = ss.make_solver(ss.Config(nx=32, ny=32, nz=32, velocity=1500.0, pulse_sigma=4.0, variant=variant))
+    for _ in range(steps):
+        s.step()
+    out = s.download()
+    ref = gd[f"naive{steps}_nofma"]
+    if variant == "naive":
+        assert np.array_equal(out, ref)
+    else:
+        rel = np.linalg.norm((out - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64))
+        assert rel <= 1e-5, rel
+    nat = gd[f"naive{steps}_native"].astype(np.float64)
+    assert np.linalg.norm(out - nat) / np.linalg.norm(nat) <= 1e-5
+
+
+def test_run_with_warmup_matches_golden(ss, dev):
+    gd = load_golden("solver_wave_32.npz")
+    s = ss.make_solver(ss.Config(nx=32, ny=32, nz=32, velocity=1500.0, pulse_sigma=4.0, variant="naive"))
+    r = s.run(5, 2)
+    assert r.label == "wave/naive" and r.time_ms > 0 and r.grid_size == 32
+    assert np.array_equal(s.download(), gd["run5_warm2_nofma"])
+
+
+@pytest.mark.parametrize("variant", ["naive", "tiled"])
+def test_heat_solver_matches_reference_golden(ss, dev, variant):
+    gd = load_golden("solver_heat_48.npz")
+    s = ss.make_solver(ss.Config(nx=48, ny=48, nz=48, dx=1.0, dy=1.0, dz=1.0, velocity=500.0,
+                                 diffusivity=float(np.float32(1e-3)), pulse_sigma=5.0, physics="heat",
+                                 variant=variant))
+    assert s.name() == f"heat/{variant}"
+    for _ in range(20):
+        s.step()
+    out = s.download()
+    ref = gd["heat20_nofma"]
+    if variant == "naive":
+        assert np.array_equal(out, ref)
+    else:
+        assert np.linalg.norm((out - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64)) <= 1e-5
+
+
+@pytest.mark.parametrize("variant", ["naive", "tiled"])
+def test_acoustic128_matches_reference_golden(ss, dev, variant, golden_meta):
+    """128^3, h=10, v=2000, dt=1e-3, sigma=3, 200 steps."""
+    gd = load_golden("acoustic_128.npz")
+    p0 = ss.host_gaussian((128, 128, 128), 0.5, 0.5, 0.5, 3.0)
+    assert sha(p0) == golden_meta["acoustic128_nofma"]["sha_init"]
+    vel2 = np.full_like(p0, np.float32(2000.0) * np.float32(2000.0))
+    grid = ss.GridDims.cube(128, np.float32(10.0), np.float32(1e-3))
+    out = ring_run(ss, variant, p0, vel2, grid, 200, dev)
+    if variant == "naive":
+        assert sha(out) == golden_meta["acoustic128_nofma"]["sha_final"]
+    mid = gd["mid_z_nofma"].astype(np.float64)
+    assert np.linalg.norm(out[64] - mid) / np.linalg.norm(mid) <= 1e-5
+    nat = gd["mid_z_native"].astype(np.float64)
+    assert np.linalg.norm(out[64] - nat) / np.linalg.norm(nat) <= 1e-5
+
+
+def test_acoustic128_fp64_matches_reference_golden(ss, dev, golden_meta):
+    from oracle import Oracle
+
+    p0 = Oracle().gaussian(128, sigma=3.0, dtype=np.float64)
+    vel2 = np.full_like(p0, 2000.0 * 2000.0)
+    grid = ss.GridDims.cube(128, 10.0, 1e-3)
+    for variant in ("naive", "tiled"):
+        out = ring_run(ss, variant, p0, vel2, grid, 200, dev)
+        assert sha(out) == golden_meta["acoustic128_f64"]["sha_final"]
+
+
+# ── against the oracle on ragged shapes, seeded inputs ───────────────────────
+SHAPES = [(36, 52, 40), (40, 44, 72), (24, 24, 24), (33, 29, 136), (20, 61, 12), (28, 30, 30)]
+
+
+def _inputs(oracle, shape, seed):
+    p0 = oracle.gaussian(0, 0.45, 0.55, 0.5, 3.0, shape=shape)
+    vel2 = oracle.layered_noise_vel2(shape, n_layers=5, vmin=1500.0, vmax=4000.0, noise=0.05, seed=seed)
+    return p0, vel2
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_naive_bit_exact_vs_oracle(ss, dev, oracle, shape):
+    p0, vel2 = _inputs(oracle, shape, 11)
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
+                       np.float32(8e-4))
+    ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 6, grid.dy, grid.dz)
+    assert np.array_equal(ring_run(ss, "naive", p0, vel2, grid, 6, dev), ref)
+
+
+@pytest.mark.parametrize("shape", [s for s in SHAPES if s[2] % 4 == 0])
+@pytest.mark.parametrize("ty,n_chunks,xcd", [(16, 0, -1), (24, 1, 0), (32, 3, 1), (16, 5, 0)])
+def test_tiled_exact_bit_exact_vs_oracle(ss, dev, oracle, shape, ty, n_chunks, xcd):
+    p0, vel2 = _inputs(oracle, shape, 12)
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
+                       np.float32(8e-4))
+    ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 5, grid.dy, grid.dz)
+    out = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=True, ty=ty, n_chunks=n_chunks, xcd_remap=xcd)
+    assert np.array_equal(out, ref)
+    fused = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=False, ty=ty, n_chunks=n_chunks,
+                     xcd_remap=xcd)
+    assert oracle.rel_l2_interior(fused, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tiled_default_vs_oracle(ss, dev, oracle, shape):
+    """launch_tiled takes any shape (nx % 4 != 0 runs the coalesced kernel)."""
+    p0, vel2 = _inputs(oracle, shape, 13)
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(10.0), np.float32(10.0),
+                       np.float32(1e-3))
+    ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 8)
+    out = ring_run(ss, "tiled", p0, vel2, grid, 8, dev)
+    assert oracle.rel_l2_interior(out, ref) <= 1e-5
+    if shape[2] % 4 != 0:
+        assert np.array_equal(out, ref)
+
+
+def test_layered_model_128_50_steps(ss, dev, oracle):
+    """The 512^3 workload's velocity model at 128^3: 16 layers, +-5 % noise."""
+    shape = (128, 128, 128)
+    p0 = ss.host_gaussian(shape, 0.5, 0.5, 0.5, 3.0)
+    vel2 = ss.host_layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.05, 2)
+    assert np.array_equal(vel2, oracle.layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.05, 2))
+    grid = ss.GridDims.cube(128, np.float32(10.0), np.float32(1e-3))
+    ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 50)
+    assert np.array_equal(ring_run(ss, "naive", p0, vel2, grid, 50, dev), ref)
+    assert oracle.rel_l2_interior(ring_run(ss, "tiled", p0, vel2, grid, 50, dev), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_heat_tiled_vs_oracle(ss, dev, oracle, exact):
+    import torch
+
+    shape = (40, 36, 48)
+    t0 = oracle.heat_noise_cube(shape, side=8, hot=10.0, seed=2)
+    dte = oracle.heat_dt_eff(np.float32(5.0))
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(10.0), np.float32(10.0), dte)
+    ref = oracle.heat_run(t0, np.float32(10.0), dte, np.float32(1.0), 10)
+    a, b = to_dev(t0, dev), torch.zeros(shape, device=dev)
+    for _ in range(10):
+        ss.heat_step(a, b, 1.0, grid, exact=exact)
+        a, b = b, a
+    out = a.cpu().numpy()
+    if exact:
+        assert np.array_equal(out, ref)
+    else:
+        assert oracle.rel_l2_interior(out, ref) <= 1e-5
+    a, b = to_dev(t0, dev), torch.zeros(shape, device=dev)
+    for _ in range(10):
+        ss.heat_step(a, b, 1.0, grid, variant="naive")
+        a, b = b, a
+    assert np.array_equal(a.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("variant", ["naive", "tiled"])
+def test_halo_never_written(ss, dev, oracle, variant):
+    import torch
+
+    shape = (36, 44, 40)
+    p0, vel2 = _inputs(oracle, shape, 14)
+    grid = ss.GridDims(40, 44, 36, np.float32(10.0), np.float32(10.0), np.float32(10.0), np.float32(1e-3))
+    nxt = torch.full(shape, 7.5, device=dev)
+    ss.launch(variant, to_dev(p0, dev), to_dev(p0, dev), to_dev(vel2, dev), nxt, grid)
+    out = nxt.cpu().numpy()
+    mask = np.ones(shape, bool)
+    mask[R:-R, R:-R, R:-R] = False
+    assert np.all(out[mask] == np.float32(7.5))
+    ref = np.full(shape, np.float32(7.5))
+    oracle.wave_step(p0, p0, vel2, ref, grid.dx, grid.dt)
+    assert oracle.rel_l2_interior(out, ref) <= 1e-6
+
+
+def test_runs_on_caller_stream(ss, dev, oracle):
+    import torch
+
+    shape = (32, 32, 32)
+    p0, vel2 = _inputs(oracle, shape, 15)
+    grid = ss.GridDims.cube(32, np.float32(10.0), np.float32(1e-3))
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a, v = to_dev(p0, dev), to_dev(vel2, dev)
+        n = torch.zeros_like(a)
+        ss.launch_tiled(a, a, v, n, grid, stream=s)
+    s.synchronize()
+    ref = np.zeros_like(p0)
+    oracle.wave_step(p0, p0, vel2, ref, grid.dx, grid.dt)
+    assert oracle.rel_l2_interior(n.cpu().numpy(), ref) <= 1e-6
+
+
+def test_argument_errors(ss, dev):
+    import torch
+
+    grid = ss.GridDims.cube(16, 10.0, 1e-3)
+    a = torch.zeros((16, 16, 16), device=dev)
+    with pytest.raises(TypeError):
+        ss.launch_naive(a.cpu(), a, a, torch.zeros_like(a), grid)
+    with pytest.raises(ValueError):
+        ss.launch_naive(a, a, a, torch.zeros((16, 16, 15), device=dev), ss.GridDims.cube(16))
+    with pytest.raises(ss.StencilError):
+        ss.launch_naive(a, a, a, a, grid)  # p_next aliases p_cur
+    with pytest.raises(TypeError):
+        ss.launch_tiled(a.double(), a, a, a, grid)
+
+
+# ── full-size properties ─────────────────────────────────────────────────────
+@pytest.mark.parametrize("n", [512, 1024])
+def test_fullsize_tiled_equals_naive(ss, dev, n):
+    """At 512^3 and 1024^3: exact-mode tiled == naive bit for bit over 3 steps,
+    fused tiled within 1e-5 rel-L2 of naive, halo untouched, x-mirror symmetry
+    of a centred pulse in a z-layered medium."""
+    import torch
+
+    shape = (n, n, n)
+    p0 = to_dev(ss.host_gaussian(shape, 0.5, 0.5, 0.5, 3.0), dev)
+    vel2 = to_dev(ss.host_layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.0, 2), dev)
+    grid = ss.GridDims.cube(n, np.float32(10.0), np.float32(1e-3))
+
+    def run(kind, steps):
+        bufs = [p0.clone(), p0.clone(), torch.full(shape, 3.25, device=dev)]
+        i = 0
+        for _ in range(steps):
+            args = (bufs[(i + 1) % 3], bufs[i], vel2, bufs[(i + 2) % 3], grid)
+            if kind == "naive":
+                ss.launch_naive(*args)
+            else:
+                ss.launch_tiled_ex(*args, exact=(kind == "exact"))
+            i = (i + 1) % 3
+        torch.cuda.synchronize()
+        return bufs[(i + 1) % 3], bufs
+
+    a, _ = run("naive", 3)
+    b, _ = run("exact", 3)
+    assert torch.equal(a, b)
+    del b
+    c, _ = run("fused", 3)
+    num = torch.linalg.vector_norm((c - a).double())
+    assert (num / torch.linalg.vector_norm(a.double())).item() <= 1e-5
+    # the buffer that started at 3.25 has been written once (step 3): halo intact
+    _, bufs = run("fused", 3)
+    w = bufs[2]
+    assert torch.all(w[:R] == 3.25) and torch.all(w[:, :, -R:] == 3.25) and torch.all(w[:, :R] == 3.25)
+    assert torch.equal(a, torch.flip(a, dims=[2]))  # constant-in-x medium, centred pulse
