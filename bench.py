@@ -36,10 +36,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512, help="cells per axis (halo included)")
-    ap.add_argument("--variant", default="tiled", choices=["tiled", "naive", "tiled_exact"])
+    ap.add_argument("--variant", default="tiled", choices=["tiled", "naive", "tiled_fused"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the side measurements")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run 2 STREAM copies of one field (known bytes) for counter scaling")
+    ap.add_argument("--tune-csv", default=None, help="write the autotuner sweep to this CSV")
     return ap.parse_args()
 
 
@@ -73,8 +77,8 @@ def main():
     def launcher(variant):
         if variant == "naive":
             return lambda c, p, v, o: ss.launch_naive(c, p, v, o, grid)
-        if variant == "tiled_exact":
-            return lambda c, p, v, o: ss.launch_tiled_ex(c, p, v, o, grid, exact=True)
+        if variant == "tiled_fused":
+            return lambda c, p, v, o: ss.launch_tiled_ex(c, p, v, o, grid, exact=False)
         return lambda c, p, v, o: ss.launch_tiled(c, p, v, o, grid)
 
     def timed_run(variant, steps, warmup):
@@ -96,6 +100,16 @@ def main():
         out = bufs[(i + 1) % 3]
         del bufs
         return ms, out
+
+    tuned = None
+    if args.variant == "tiled" and not args.no_tune:
+        # One-time autotune of the launch shape for this grid size (outside the
+        # timed region; the launcher caches the result).
+        tb = [p0.clone(), p0.clone(), torch.zeros_like(p0)]  # distinct buffers, as in the run
+        tuned = ss.tune_tiled(tb[1], tb[0], vel2, tb[2], grid, reps=5, csv_path=args.tune_csv)
+        del tb
+        print(f"[bench] tuned shape {tuned}", file=sys.stderr, flush=True)
+    shape_used, _ = ss.tuned_shape(grid)
 
     ms, out = timed_run(args.variant, args.steps, args.warmup)
     gpts = interior / (ms * 1e-3) / 1e9
@@ -121,7 +135,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"acoustic {n}^3 FP32 leapfrog, 8th-order Laplacian, 16-layer +-5% velocity, "
                                f"Gaussian sigma=3, h=10 m, dt=1e-3 s",
-                   "grid": n, "interior_points": interior, "kernel": args.variant},
+                   "grid": n, "interior_points": interior, "kernel": args.variant,
+                   "tile_shape": shape_used if args.variant == "tiled" else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": alg_bytes},
@@ -138,14 +153,14 @@ def main():
         extras = {}
         # Same steps through the naive (reference-association) kernel: the
         # measured run's result must agree with it (tolerance 1e-5 rel-L2).
-        ms_n, out_n = timed_run("naive", args.steps, 0)
+        ms_n, out_n = timed_run("naive", args.steps, args.warmup)
         rel = (torch.linalg.vector_norm((out - out_n).double()) /
                torch.linalg.vector_norm(out_n.double())).item()
         extras["naive_gpts"] = round(interior / (ms_n * 1e-3) / 1e9, 2)
         extras["rel_l2_vs_naive_after_steps"] = rel
         del out_n
-        ms_e, _ = timed_run("tiled_exact", min(args.steps, 200), 5)
-        extras["tiled_exact_gpts"] = round(interior / (ms_e * 1e-3) / 1e9, 2)
+        ms_e, _ = timed_run("tiled_fused", min(args.steps, 200), 5)
+        extras["tiled_fused_gpts"] = round(interior / (ms_e * 1e-3) / 1e9, 2)
         # Heat step (dedicated single-level kernel): 8 B / point.
         a, b = p0.clone(), torch.zeros_like(p0)
         hgrid = ss.GridDims.cube(n, h, np.float32(np.sqrt(np.float64(5.0))))
@@ -180,8 +195,35 @@ def main():
         cms = e0.elapsed_time(e1) / 20
         extras["stream_copy_gbs"] = round(8.0 * src.numel() / (cms * 1e-3) / 1e9, 1)
         del src, dst
+        # 3-read / 1-write streaming kernel (the stencil's byte mix, no reuse):
+        # the practical ceiling this HBM delivers for 16 B / point.
+        src3 = torch.ones(3 * n * n * n, device=dev)
+        dst = torch.empty(n * n * n, device=dev)
+        for _ in range(3):
+            L.stencil_stream_f32(3, dst.data_ptr(), src3.data_ptr(), dst.numel(), stream)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            L.stencil_stream_f32(3, dst.data_ptr(), src3.data_ptr(), dst.numel(), stream)
+        e1.record()
+        torch.cuda.synchronize()
+        tms = e0.elapsed_time(e1) / 20
+        extras["stream_3r1w_gbs"] = round(16.0 * dst.numel() / (tms * 1e-3) / 1e9, 1)
+        del src3, dst
         result["extras"] = extras
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
+
+    if args.calibrate:
+        # Known-size copy: n^3 floats read + n^3 floats written per launch, the
+        # reference the FETCH_SIZE / WRITE_SIZE counters are scaled against.
+        src = torch.ones(n * n * n, device=dev)
+        dst = torch.empty_like(src)
+        stream = torch.cuda.current_stream().cuda_stream
+        for _ in range(2):
+            ss.lib().stencil_stream_f32(0, dst.data_ptr(), src.data_ptr(), src.numel(), stream)
+        torch.cuda.synchronize()
+        result["calibration_copy_bytes"] = 4 * src.numel()
+        del src, dst
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p0_h, vel2_h, h, dt, interior, args.cpu_seconds)

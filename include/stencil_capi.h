@@ -67,7 +67,9 @@ int stencil_launch_naive_f32(const float* p_cur, const float* p_prev, const floa
                              float* p_next, const stencil_grid* grid, void* hip_stream);
 /* Replaces launch_tiled (reference include/stencil/kernels.cuh:54-55,
  * src/kernels/stencil_tiled.cu:147-168): same step, LDS-staged plane + z
- * register queue, FMA-fused weights (within 1e-5 rel-L2 of the reference). */
+ * register queue, same association (bit-identical to launch_naive).  Shapes
+ * the vector kernel cannot take (nx % 4 != 0, unaligned bases) run the
+ * coalesced kernel. */
 int stencil_launch_tiled_f32(const float* p_cur, const float* p_prev, const float* vel2,
                              float* p_next, const stencil_grid* grid, void* hip_stream);
 /* Replaces launch(KernelVariant, ...) (reference include/stencil/kernels.cuh:68-83). */
@@ -89,8 +91,8 @@ int stencil_launch_f64(int variant, const double* p_cur, const double* p_prev,
  * p_prev = p_cur, then a full-field D2D copy) with one single-level kernel:
  *   t_next = t + (dt^2 alpha) Lap(t)
  * where grid->dt is the heat solver's dt_eff = (float)sqrt((double)dt)
- * (heat_diffusion.cpp:86-87).  Bit-identical to the reference for
- * STENCIL_VARIANT_NAIVE; TILED uses the fused-weight form. */
+ * (heat_diffusion.cpp:86-87).  Bit-identical to the reference for every
+ * variant. */
 int stencil_heat_step_f32(const float* t, float* t_next, float alpha, const stencil_grid* grid,
                           int variant, void* hip_stream);
 
@@ -105,21 +107,52 @@ double stencil_heat_bytes_per_step(const stencil_grid* grid, int bytes_per_value
 
 /* ---- Extensions (no counterpart in the reference) ---- */
 
-/* The tiled step with an explicit launch shape, for the autotuner and for
- * measurement: exact != 0 selects the reference association; ty in
- * {16, 24, 32} rows per tile (0 = default), n_chunks z-chunks per column
- * (0 = automatic), xcd_remap 1/0/-1 = on/off/automatic.  Returns
- * STENCIL_ERR_UNSUPPORTED when nx % 4 != 0 or a base is not 16-byte aligned
- * (stencil_launch_tiled_f32 then runs the naive kernel instead). */
+/* Launch shape of the tiled kernel (autotuner / measurement).  Zero or -1
+ * fields are chosen by the library.  (tile_x4, tile_y) must be one of the
+ * built shapes: (16,16) (16,32) (32,8) (32,16) (64,4) (64,8) -- a tile is
+ * 4*tile_x4 cells wide and tile_y rows tall, one thread per 4 cells. */
+typedef struct stencil_tile_shape {
+    int tile_x4;     /* float4 columns per tile row (0 = default) */
+    int tile_y;      /* rows per tile (0 = default) */
+    int n_chunks;    /* z-chunks per column (0 = automatic) */
+    int xcd_remap;   /* 1 on, 0 off, -1 automatic */
+    int nontemporal; /* 1 non-temporal p_prev/vel2/p_next, 0 cached, -1 automatic */
+} stencil_tile_shape;
+
+/* The tiled step with an explicit launch shape (NULL = defaults).  exact != 0
+ * selects the reference association (the default of stencil_launch_tiled_f32);
+ * exact == 0 an FMA form with combined weights c_r/h^2 (fewer VALU ops; its
+ * rounding drifts from the reference by ~1e-5 relative L2 over 50-200 steps,
+ * so it is opt-in).  Returns STENCIL_ERR_UNSUPPORTED when nx % 4 != 0 or a
+ * base is not 16-byte aligned (stencil_launch_tiled_f32 then runs the naive
+ * kernel instead). */
 int stencil_launch_tiled_ex_f32(const float* p_cur, const float* p_prev, const float* vel2,
-                                float* p_next, const stencil_grid* grid, int exact, int ty,
-                                int n_chunks, int xcd_remap, void* hip_stream);
+                                float* p_next, const stencil_grid* grid, int exact,
+                                const stencil_tile_shape* tile, void* hip_stream);
 int stencil_heat_step_ex_f32(const float* t, float* t_next, float alpha, const stencil_grid* grid,
-                             int exact, int ty, int n_chunks, int xcd_remap, void* hip_stream);
+                             int exact, const stencil_tile_shape* tile, void* hip_stream);
+
+/* Autotuner (replaces the reference's hand-run tuning_results/ sweeps): times
+ * every built tile shape x z-chunk count x streaming mode of the tiled kernel
+ * (reps launches each, hipEvents on hip_stream) on the caller's buffers --
+ * p_next / t_next are overwritten -- and caches the fastest for this grid
+ * size; later tiled launches with a NULL shape on the same dims use it.
+ * Writes the sweep as CSV to csv_path when it is not NULL.  Synchronises. */
+int stencil_tune_tiled_f32(const float* p_cur, const float* p_prev, const float* vel2,
+                           float* p_next, const stencil_grid* grid, int reps,
+                           stencil_tile_shape* best, const char* csv_path, void* hip_stream);
+int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* grid, int reps,
+                          stencil_tile_shape* best, const char* csv_path, void* hip_stream);
+/* The shape a NULL-shape launch would use for these dims (kind 0 wave, 1
+ * heat): returns 1 if it came from the tuner cache, 0 for the default. */
+int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
+void stencil_clear_tuning(void);
 
 /* HBM calibration (the reference's tests/perf/bench_stream.cu:53-101):
  * op 0 = copy src -> dst, 1 = read src (dst receives at most 2048 partial
- * sums), 2 = write dst := 1.  n floats, n % 4 == 0, 16-byte aligned. */
+ * sums), 2 = write dst := 1, 3 = dst = src[0:n] + src[n:2n] + src[2n:3n]
+ * (the stencil's 3-read / 1-write mix; src holds 3n floats).  n floats per
+ * array, n % 4 == 0, 16-byte aligned. */
 int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* hip_stream);
 
 /* Seeded host-side input generators (OpenMP), the same values on every

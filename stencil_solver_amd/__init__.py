@@ -21,6 +21,11 @@ from .api import (  # noqa: F401
     launch_tiled,
     launch_tiled_ex,
     make_solver,
+    tile_shape,
+    tune_heat,
+    tune_tiled,
+    tuned_shape,
+    clear_tuning,
     upload_fd_coefficients,
     wave_bytes_per_step,
 )

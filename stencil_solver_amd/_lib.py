@@ -42,6 +42,11 @@ class stencil_config(C.Structure):
                 ("seed", C.c_uint64), ("fp64", C.c_int)]
 
 
+class stencil_tile_shape(C.Structure):
+    _fields_ = [("tile_x4", C.c_int), ("tile_y", C.c_int), ("n_chunks", C.c_int), ("xcd_remap", C.c_int),
+                ("nontemporal", C.c_int)]
+
+
 class stencil_perf(C.Structure):
     _fields_ = [("time_ms", C.c_double), ("bandwidth_gbs", C.c_double), ("gflops", C.c_double),
                 ("model_bandwidth_gbs", C.c_double), ("grid_size", C.c_uint64),
@@ -67,9 +72,13 @@ SIGNATURES = {
     "stencil_heat_step_f64": (_I, [_P, _P, C.c_double, _G, _I, _P]),
     "stencil_wave_bytes_per_step": (C.c_double, [_G, _I]),
     "stencil_heat_bytes_per_step": (C.c_double, [_G, _I]),
-    "stencil_launch_tiled_ex_f32": (_I, [_P, _P, _P, _P, _G, _I, _I, _I, _I, _P]),
-    "stencil_heat_step_ex_f32": (_I, [_P, _P, C.c_float, _G, _I, _I, _I, _I, _P]),
+    "stencil_launch_tiled_ex_f32": (_I, [_P, _P, _P, _P, _G, _I, C.POINTER(stencil_tile_shape), _P]),
+    "stencil_heat_step_ex_f32": (_I, [_P, _P, C.c_float, _G, _I, C.POINTER(stencil_tile_shape), _P]),
     "stencil_stream_f32": (_I, [_I, _P, _P, C.c_uint64, _P]),
+    "stencil_tune_tiled_f32": (_I, [_P, _P, _P, _P, _G, _I, C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
+    "stencil_tune_heat_f32": (_I, [_P, _P, _G, _I, C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
+    "stencil_tuned_shape": (_I, [_G, _I, C.POINTER(stencil_tile_shape)]),
+    "stencil_clear_tuning": (None, []),
     "stencil_host_init_gaussian_f32": (_I, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
                                             C.c_float, C.c_float, C.c_float]),
     "stencil_host_layered_noise_vel2_f32": (_I, [_P, C.c_uint64, C.c_uint64, C.c_uint64, _I,

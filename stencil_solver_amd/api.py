@@ -20,7 +20,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import StencilError, check, lib, stencil_config, stencil_grid, stencil_perf
+from ._lib import StencilError, check, lib, stencil_config, stencil_grid, stencil_perf, stencil_tile_shape
 
 VARIANTS = {"naive": 0, "tiled": 1, "multigpu": 2}
 RADIUS = 4
@@ -131,16 +131,22 @@ def launch(variant, p_cur, p_prev, vel2, p_next, grid: GridDims, stream=None) ->
         raise ValueError(f"unknown kernel variant {variant!r}")
 
 
-def launch_tiled_ex(p_cur, p_prev, vel2, p_next, grid: GridDims, exact=False, ty=0, n_chunks=0,
-                    xcd_remap=-1, stream=None) -> None:
+def tile_shape(tx4=0, ty=0, n_chunks=0, xcd_remap=-1, nontemporal=-1) -> stencil_tile_shape:
+    """Launch shape of the tiled kernel (see include/stencil_capi.h)."""
+    return stencil_tile_shape(int(tx4), int(ty), int(n_chunks), int(xcd_remap), int(nontemporal))
+
+
+def launch_tiled_ex(p_cur, p_prev, vel2, p_next, grid: GridDims, exact=True, tx4=0, ty=0, n_chunks=0,
+                    xcd_remap=-1, nontemporal=-1, stream=None) -> None:
     """The tiled step with an explicit launch shape (no reference counterpart)."""
     L = lib()
+    shp = tile_shape(tx4, ty, n_chunks, xcd_remap, nontemporal)
     _wave(L.stencil_launch_tiled_ex_f32, None, p_cur, p_prev, vel2, p_next, grid, stream,
-          "launch_tiled_ex", extra=(int(bool(exact)), int(ty), int(n_chunks), int(xcd_remap)))
+          "launch_tiled_ex", extra=(int(bool(exact)), C.byref(shp)))
 
 
 def heat_step(t, t_next, alpha: float, grid: GridDims, variant="tiled", stream=None, exact=None,
-              ty=0, n_chunks=0, xcd_remap=-1) -> None:
+              tx4=0, ty=0, n_chunks=0, xcd_remap=-1, nontemporal=-1) -> None:
     """t_next = t + (dt^2 alpha) Lap(t); grid.dt is the heat dt_eff."""
     import torch
 
@@ -156,8 +162,55 @@ def heat_step(t, t_next, alpha: float, grid: GridDims, variant="tiled", stream=N
         check(L.stencil_heat_step_f32(*ptrs, float(alpha), C.byref(g), _variant(variant), _stream(stream)),
               "heat_step")
     else:
-        check(L.stencil_heat_step_ex_f32(*ptrs, float(alpha), C.byref(g), int(bool(exact)), int(ty),
-                                         int(n_chunks), int(xcd_remap), _stream(stream)), "heat_step_ex")
+        shp = tile_shape(tx4, ty, n_chunks, xcd_remap, nontemporal)
+        check(L.stencil_heat_step_ex_f32(*ptrs, float(alpha), C.byref(g), int(bool(exact)), C.byref(shp),
+                                         _stream(stream)), "heat_step_ex")
+
+
+def _shape_dict(shp) -> dict:
+    return {"tile_x": 4 * shp.tile_x4, "tile_y": shp.tile_y, "n_chunks": shp.n_chunks,
+            "xcd_remap": shp.xcd_remap, "nontemporal": shp.nontemporal}
+
+
+def tune_tiled(p_cur, p_prev, vel2, p_next, grid: GridDims, reps: int = 5, csv_path=None, stream=None) -> dict:
+    """Autotune the tiled wave kernel for this grid size (p_next is
+    overwritten); later launch_tiled calls on these dims use the result."""
+    import torch
+
+    shp = stencil_tile_shape()
+    ptrs = [_dev_ptr(t, grid, torch.float32, n) for t, n in
+            ((p_cur, "p_cur"), (p_prev, "p_prev"), (vel2, "vel2"), (p_next, "p_next"))]
+    g = grid.to_c()
+    path = str(csv_path).encode() if csv_path else None
+    check(lib().stencil_tune_tiled_f32(*ptrs, C.byref(g), int(reps), C.byref(shp), path, _stream(stream)),
+          "tune_tiled")
+    return _shape_dict(shp)
+
+
+def tune_heat(t, t_next, grid: GridDims, reps: int = 5, csv_path=None, stream=None) -> dict:
+    import torch
+
+    shp = stencil_tile_shape()
+    ptrs = [_dev_ptr(x, grid, torch.float32, n) for x, n in ((t, "t"), (t_next, "t_next"))]
+    g = grid.to_c()
+    path = str(csv_path).encode() if csv_path else None
+    check(lib().stencil_tune_heat_f32(*ptrs, C.byref(g), int(reps), C.byref(shp), path, _stream(stream)),
+          "tune_heat")
+    return _shape_dict(shp)
+
+
+def tuned_shape(grid: GridDims, kind: str = "wave"):
+    """(shape dict, from_tuner) a NULL-shape launch would use on these dims."""
+    shp = stencil_tile_shape()
+    g = grid.to_c()
+    r = lib().stencil_tuned_shape(C.byref(g), {"wave": 0, "heat": 1}[kind], C.byref(shp))
+    if r < 0 or r > 1:
+        check(r, "tuned_shape")
+    return _shape_dict(shp), bool(r)
+
+
+def clear_tuning() -> None:
+    lib().stencil_clear_tuning()
 
 
 def wave_bytes_per_step(grid: GridDims, bytes_per_value: int = 4) -> float:
@@ -312,5 +365,6 @@ def make_solver(cfg: Config) -> Solver:
 
 
 __all__ = ["GridDims", "Config", "PerfResult", "Solver", "StencilError", "make_solver", "upload_fd_coefficients",
-           "launch_naive", "launch_tiled", "launch", "launch_tiled_ex", "heat_step", "wave_bytes_per_step",
+           "launch_naive", "launch_tiled", "launch", "launch_tiled_ex", "tile_shape", "tune_tiled", "tune_heat",
+           "tuned_shape", "clear_tuning", "heat_step", "wave_bytes_per_step",
            "heat_bytes_per_step", "host_gaussian", "host_layered_noise_vel2", "host_noise_cube", "VARIANTS"]

@@ -84,18 +84,39 @@ int stencil_launch_naive_f32(const float* p_cur, const float* p_prev, const floa
                                      scalars_of<float>(grid), static_cast<hipStream_t>(stream)));
 }
 
+namespace {
+// An explicit shape, else the autotuned one for these dims (kind 0 wave,
+// 1 heat), else the library default.
+bool shape_of(const stencil_tile_shape* t, const Dims& d, int kind, ZqShape* out) {
+    ZqShape s;
+    if (!t) {
+        if (zq_cached_shape(d, kind, &s)) {
+            *out = s;
+            return true;
+        }
+    } else {
+        if ((t->tile_x4 != 0 || t->tile_y != 0) && !zq_shape_valid(t->tile_x4, t->tile_y)) return false;
+        if (t->n_chunks < 0) return false;
+        s.txv = t->tile_x4;
+        s.ty = t->tile_y;
+        s.n_chunks = t->n_chunks;
+        s.xcd_remap = t->xcd_remap;
+        s.nt = t->nontemporal;
+    }
+    *out = s;
+    return true;
+}
+}  // namespace
+
 int stencil_launch_tiled_ex_f32(const float* p_cur, const float* p_prev, const float* vel2,
-                                float* p_next, const stencil_grid* grid, int exact, int ty,
-                                int n_chunks, int xcd_remap, void* stream) {
+                                float* p_next, const stencil_grid* grid, int exact,
+                                const stencil_tile_shape* tile, void* stream) {
     if (!wave_args_ok(p_cur, p_prev, vel2, p_next, grid)) return STENCIL_ERR_INVALID;
     const Dims d = dims_of(grid);
     const void* ptrs[4] = {p_cur, p_prev, vel2, p_next};
-    if (!zq_supported(d, ptrs, 4)) return STENCIL_ERR_UNSUPPORTED;
-    if (ty != 0 && ty != 16 && ty != 24 && ty != 32) return STENCIL_ERR_INVALID;
     ZqShape shape;
-    shape.ty = ty;
-    shape.n_chunks = n_chunks;
-    shape.xcd_remap = xcd_remap;
+    if (!shape_of(tile, d, 0, &shape)) return STENCIL_ERR_INVALID;
+    if (!zq_supported(d, ptrs, 4)) return STENCIL_ERR_UNSUPPORTED;
     return hip_status(wave_zq_f32(p_cur, p_prev, vel2, p_next, d, scalars_of<float>(grid),
                                   exact != 0, shape, static_cast<hipStream_t>(stream)));
 }
@@ -103,7 +124,7 @@ int stencil_launch_tiled_ex_f32(const float* p_cur, const float* p_prev, const f
 int stencil_launch_tiled_f32(const float* p_cur, const float* p_prev, const float* vel2,
                              float* p_next, const stencil_grid* grid, void* stream) {
     const int st =
-        stencil_launch_tiled_ex_f32(p_cur, p_prev, vel2, p_next, grid, 0, 0, 0, -1, stream);
+        stencil_launch_tiled_ex_f32(p_cur, p_prev, vel2, p_next, grid, 1, nullptr, stream);
     if (st != STENCIL_ERR_UNSUPPORTED) return st;
     // nx % 4 != 0 or unaligned bases: the coalesced kernel takes any shape.
     return stencil_launch_naive_f32(p_cur, p_prev, vel2, p_next, grid, stream);
@@ -144,17 +165,14 @@ int stencil_launch_f64(int variant, const double* p_cur, const double* p_prev,
 }
 
 int stencil_heat_step_ex_f32(const float* t, float* t_next, float alpha, const stencil_grid* grid,
-                             int exact, int ty, int n_chunks, int xcd_remap, void* stream) {
+                             int exact, const stencil_tile_shape* tile, void* stream) {
     if (!t || !t_next || t == t_next || !valid_grid(grid)) return STENCIL_ERR_INVALID;
     const Dims d = dims_of(grid);
     const void* ptrs[2] = {t, t_next};
-    if (!zq_supported(d, ptrs, 2)) return STENCIL_ERR_UNSUPPORTED;
-    if (ty != 0 && ty != 16 && ty != 24 && ty != 32) return STENCIL_ERR_INVALID;
-    const Scalars<float> s = scalars_of<float>(grid);
     ZqShape shape;
-    shape.ty = ty;
-    shape.n_chunks = n_chunks;
-    shape.xcd_remap = xcd_remap;
+    if (!shape_of(tile, d, 1, &shape)) return STENCIL_ERR_INVALID;
+    if (!zq_supported(d, ptrs, 2)) return STENCIL_ERR_UNSUPPORTED;
+    const Scalars<float> s = scalars_of<float>(grid);
     return hip_status(heat_zq_f32(t, t_next, d, s, s.dt2 * alpha, exact != 0, shape,
                                   static_cast<hipStream_t>(stream)));
 }
@@ -168,7 +186,7 @@ int stencil_heat_step_f32(const float* t, float* t_next, float alpha, const sten
     // dt2 * vel2[c] of the reference with vel2 == alpha, formed in float.
     const float coef = s.dt2 * alpha;
     if (variant != STENCIL_VARIANT_NAIVE) {
-        const int st = stencil_heat_step_ex_f32(t, t_next, alpha, grid, 0, 0, 0, -1, stream);
+        const int st = stencil_heat_step_ex_f32(t, t_next, alpha, grid, 1, nullptr, stream);
         if (st != STENCIL_ERR_UNSUPPORTED) return st;
     }
     return hip_status(

@@ -33,15 +33,26 @@ Scalars<T> make_scalars(T dx, T dy, T dz, T dt) {
 // Launch shape of the LDS + z-register-queue kernel.  Zero fields = choose
 // automatically (see zq_resolve).
 struct ZqShape {
-    int ty = 0;        // tile rows: 16, 24 or 32 (tile is 64 x ty; 16*ty threads)
+    int txv = 0;       // tile width in float4 (tile is 4*txv x ty cells, txv*ty threads)
+    int ty = 0;        // tile rows; (txv, ty) must be one of the built shapes
     int n_chunks = 0;  // z-chunks per column (each re-reads 8 halo planes)
     int xcd_remap = -1;  // 1 = XCD-contiguous tile order, 0 = off, -1 = auto
+    int nt = -1;       // 1 = non-temporal p_prev / vel2 / p_next, 0 = cached, -1 = auto
 };
 
-// Fills unset fields and reports whether the kernel can take these dims and
-// pointers (nx % 4 == 0, 16-byte aligned bases).  Returns false if not.
+// Whether the kernel can take these dims and pointers (nx % 4 == 0, 16-byte
+// aligned bases, in-plane offsets below 2^32).
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
+// Whether (txv, ty) is a tile shape the library is built with.
+bool zq_shape_valid(int txv, int ty);
+// Fills unset fields of a shape.
 ZqShape zq_resolve(const Dims& d, ZqShape s);
+
+// Autotuner cache (tune.cpp): the fastest shape measured for a grid size;
+// kind 0 = wave step, 1 = heat step.
+bool zq_cached_shape(const Dims& d, int kind, ZqShape* out);
+void zq_cache_put(const Dims& d, int kind, const ZqShape& s);
+void zq_cache_clear();
 
 // The acoustic step.  `exact` = the reference association (bit-identical to
 // its CPU path); otherwise FMA-fused weights.

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "device_common.hpp"
 #include "stencil_capi.h"
 
 namespace {
@@ -35,6 +36,20 @@ __global__ __launch_bounds__(kThreads) void read_kernel(const float4* __restrict
         __syncthreads();
     }
     if (threadIdx.x == 0) sink[blockIdx.x] = red[0];
+}
+
+// dst = a + b + c over three n-float arrays laid out back to back at src: the
+// stencil's 3-read / 1-write byte mix with no reuse.
+__global__ __launch_bounds__(kThreads) void triad3_kernel(const float4* __restrict__ src,
+                                                          float4* __restrict__ dst, size_t n4) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n4; i += stride) {
+        const float4 a = stencil::dev::load_stream(src + i);
+        const float4 b = stencil::dev::load_stream(src + n4 + i);
+        const float4 c = stencil::dev::load_stream(src + 2 * n4 + i);
+        stencil::dev::store_stream(dst + i, make_float4(a.x + b.x + c.x, a.y + b.y + c.y,
+                                                       a.z + b.z + c.z, a.w + b.w + c.w));
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void write_kernel(float4* __restrict__ dst, float value,
@@ -72,6 +87,11 @@ int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* s
             if (!dst || !src) return STENCIL_ERR_INVALID;
             hipLaunchKernelGGL(read_kernel, dim3(g), dim3(kThreads), 0, st,
                                reinterpret_cast<const float4*>(src), dst, n4);
+            break;
+        case 3:  // triad3: dst = src[0:n] + src[n:2n] + src[2n:3n]
+            if (!dst || !src) return STENCIL_ERR_INVALID;
+            hipLaunchKernelGGL(triad3_kernel, dim3(g), dim3(kThreads), 0, st,
+                               reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
             break;
         case 2:  // write: dst := 1.0
             if (!dst) return STENCIL_ERR_INVALID;

@@ -2,24 +2,26 @@
 // variant; replaces the reference's shared-memory tile + register pencil,
 // src/kernels/stencil_tiled.cu:37-145).
 //
-// Work decomposition.  A workgroup owns a 64 x TY tile of (x, y) columns and
-// marches through one z-chunk of planes.  Each thread owns 4 consecutive x
-// (one 16-byte float4) of one row, so a wave64 covers 4 rows of 256 bytes and
-// every global access is a dwordx4.
+// Work decomposition.  A workgroup owns a tile of (x, y) columns, 4*TXV wide
+// and TY rows tall, and marches through one z-chunk of planes.  Each thread
+// owns 4 consecutive x (one 16-byte float4) of one row, so every global access
+// is a dwordx4 and a wave64 covers 64 float4 of consecutive rows.  Tiles start
+// on x = 0 (not on the first interior column), so a tile row is whole 128-byte
+// lines and no line is written by two workgroups.
 //   * z neighbours: a 9-deep register queue of float4 per thread (planes
 //     z-4..z+4), refilled with one load per plane; the z loop is unrolled by 9
 //     so queue slots are renamed, never moved.
-//   * x / y neighbours: plane z is staged in LDS as (TY+8) rows x 18 float4
-//     (64 + 2x4 floats), written from the queue centre plus one halo float4 per
-//     loader thread (top/bottom 4 rows, left/right 4 columns; corners are never
-//     read by the star stencil).  LDS is double-buffered, so one barrier per
-//     plane suffices.
-//   * streamed fields (p_prev, vel2, p_next) are touched once, non-temporally,
-//     and the halo / stream loads for plane z+1 are issued before plane z is
-//     computed (gfx950 barriers do not drain outstanding global loads).
+//   * x / y neighbours: plane z is staged in LDS as (TY+8) rows x (TXV+2)
+//     float4, written from the queue centre plus the halo float4 (top/bottom 4
+//     rows, left/right 4 columns; corners are never read by the star stencil).
+//     LDS is double-buffered, so one barrier per plane suffices.
+//   * p_prev, vel2 and p_next are touched once per step; their loads and the
+//     halo loads for plane z+1 are issued before plane z is computed (gfx950
+//     barriers do not drain outstanding global loads).  With NT they use
+//     non-temporal accesses so they do not displace p_cur lines in L2.
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
-// write p_next once = 16 B (halo re-reads are served by L2, kept XCD-local by
-// the tile order).
+// write p_next once = 16 B (halo re-reads are L2 hits; the tile order keeps
+// neighbouring tiles on one XCD).
 #include "device_common.hpp"
 #include "launchers.hpp"
 
@@ -45,15 +47,28 @@ struct ZqArgs {
     float w0, wx[4], wy[4], wz[4];   // fused weights (heat: pre-multiplied by coef)
 };
 
-constexpr int kCols = 18;  // 16 own float4 + left/right halo float4
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
-template <int TY, int MODE>
-__global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
+template <bool NT>
+__device__ __forceinline__ float4 ld_stream(const float* p) {
+    if constexpr (NT) return dev::load_stream(reinterpret_cast<const float4*>(p));
+    return ld4(p);
+}
+template <bool NT>
+__device__ __forceinline__ void st_stream(float* p, float4 v) {
+    if constexpr (NT) dev::store_stream(reinterpret_cast<float4*>(p), v);
+    else *reinterpret_cast<float4*>(p) = v;
+}
+
+template <int TXV, int TY, int MODE, bool NT>
+__global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
     constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact);
+    constexpr int kThreads = TXV * TY;
+    constexpr int kCols = TXV + 2;  // own float4 + left/right halo float4
     constexpr int kRows = TY + 2 * kR;
-    constexpr int kHalo = 8 * 16 + 2 * TY;  // halo float4 per plane
-    static_assert(kHalo <= 16 * TY, "one halo float4 per thread at most");
+    constexpr int kHalo = 2 * kR * TXV + 2 * TY;  // halo float4 per plane
+    constexpr int kHaloPer = (kHalo + kThreads - 1) / kThreads;
     __shared__ float4 lds[2][kRows][kCols];
 
     const unsigned logical = dev::xcd_remap(blockIdx.x, gridDim.x, a.xcd_remap != 0);
@@ -62,10 +77,10 @@ __global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
     const int tile_y = static_cast<int>(rest % static_cast<unsigned>(a.tiles_y));
     const int chunk_id = static_cast<int>(rest / static_cast<unsigned>(a.tiles_y));
 
-    const int tx = static_cast<int>(threadIdx.x);
-    const int ty = static_cast<int>(threadIdx.y);
-    const int tid = ty * 16 + tx;
-    const int x0 = kR + tile_x * 64;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int tx = tid % TXV;
+    const int ty = tid / TXV;
+    const int x0 = tile_x * 4 * TXV;
     const int y0 = kR + tile_y * TY;
     const int x = x0 + 4 * tx;
     const int y = y0 + ty;
@@ -75,49 +90,52 @@ __global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
 
     const size_t plane = static_cast<size_t>(a.nx) * static_cast<size_t>(a.ny);
     const bool own_in = (x < a.nx) && (y < a.ny);
-    const bool own_out = (x < a.nx - kR) && (y < a.ny - kR);
+    const bool own_out = (x >= kR) && (x < a.nx - kR) && (y < a.ny - kR);
     const unsigned own_off = static_cast<unsigned>(y) * static_cast<unsigned>(a.nx) +
                              static_cast<unsigned>(x);
 
-    // This thread's halo float4 (if any): fixed (row, col) of the LDS tile.
-    int h_row = 0, h_col = 0;
-    bool h_in = false;
-    unsigned h_off = 0;
-    if (tid < kHalo) {
-        if (tid < 64) {
-            h_row = tid >> 4;
-            h_col = 1 + (tid & 15);
-        } else if (tid < 128) {
-            h_row = TY + kR + ((tid - 64) >> 4);
-            h_col = 1 + (tid & 15);
-        } else if (tid < 128 + TY) {
-            h_row = kR + (tid - 128);
-            h_col = 0;
+    // This thread's halo float4s: item i -> (row, col) of the LDS tile.
+    int h_row[kHaloPer], h_col[kHaloPer];
+    bool h_in[kHaloPer];
+    unsigned h_off[kHaloPer];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+        const int i = tid + k * kThreads;
+        int row = 0, col = 0;
+        if (i < kR * TXV) {
+            row = i / TXV;
+            col = 1 + i % TXV;
+        } else if (i < 2 * kR * TXV) {
+            row = TY + kR + (i - kR * TXV) / TXV;
+            col = 1 + (i - kR * TXV) % TXV;
+        } else if (i < 2 * kR * TXV + TY) {
+            row = kR + (i - 2 * kR * TXV);
+            col = 0;
         } else {
-            h_row = kR + (tid - 128 - TY);
-            h_col = kCols - 1;
+            row = kR + (i - 2 * kR * TXV - TY);
+            col = kCols - 1;
         }
-        const int hy = y0 - kR + h_row;
-        const int hx = x0 - 4 + 4 * h_col;
-        h_in = (hy < a.ny) && (hx < a.nx);
-        h_off = static_cast<unsigned>(hy) * static_cast<unsigned>(a.nx) +
-                static_cast<unsigned>(hx);
+        const int hy = y0 - kR + row;
+        const int hx = x0 - 4 + 4 * col;
+        h_row[k] = row;
+        h_col[k] = col;
+        h_in[k] = (i < kHalo) && hx >= 0 && (hy < a.ny) && (hx < a.nx);
+        h_off[k] = h_in[k] ? static_cast<unsigned>(hy) * static_cast<unsigned>(a.nx) +
+                                 static_cast<unsigned>(hx)
+                           : 0u;
     }
 
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     // Predicated loads; the zero is built in registers so the compiler cannot
     // turn the predicate into a select between a global and a private address.
     const float* const cur_base = a.cur;
     auto cur_at = [cur_base, plane](int z, unsigned off, bool ok) -> float4 {
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok) v = *reinterpret_cast<const float4*>(cur_base + static_cast<size_t>(z) * plane + off);
+        if (ok) v = ld4(cur_base + static_cast<size_t>(z) * plane + off);
         return v;
     };
     auto stream_at = [plane, own_off, own_out](const float* f, int z) -> float4 {
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own_out)
-            v = dev::load_stream(
-                reinterpret_cast<const float4*>(f + static_cast<size_t>(z) * plane + own_off));
+        if (own_out) v = ld_stream<NT>(f + static_cast<size_t>(z) * plane + own_off);
         return v;
     };
 
@@ -125,8 +143,10 @@ __global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
     float4 q[9];
 #pragma unroll
     for (int k = 0; k < 8; ++k) q[k] = cur_at(z_begin - kR + k, own_off, own_in);
-    float4 h_cur = cur_at(z_begin, h_off, h_in);
-    float4 pv_cur = zero, vv_cur = zero;
+    float4 h_cur[kHaloPer];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) h_cur[k] = cur_at(z_begin, h_off[k], h_in[k]);
+    float4 pv_cur = make_float4(0.f, 0.f, 0.f, 0.f), vv_cur = pv_cur;
     if constexpr (!kHeat) {
         pv_cur = stream_at(a.prev, z_begin);
         vv_cur = stream_at(a.vel2, z_begin);
@@ -137,85 +157,89 @@ __global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
         for (int j = 0; j < 9; ++j) {
             const int zz = zb + j;
             if (zz < z_end) {  // uniform per workgroup; a constant trip count keeps the unroll
-            // Refill the queue (plane zz+4) and prefetch plane zz+1's halo and
-            // streams; all in bounds since zz+4 <= nz-1.
-            q[(j + 8) % 9] = cur_at(zz + kR, own_off, own_in);
-            const float4 h_nxt = cur_at(zz + 1, h_off, h_in);
-            float4 pv_nxt = zero, vv_nxt = zero;
-            if constexpr (!kHeat) {
-                pv_nxt = stream_at(a.prev, zz + 1);
-                vv_nxt = stream_at(a.vel2, zz + 1);
-            }
-
-            const int b = (zz - z_begin) & 1;
-            const float4 ctr4 = q[(j + 4) % 9];
-            lds[b][ty + kR][tx + 1] = ctr4;
-            if (tid < kHalo) lds[b][h_row][h_col] = h_cur;
-            __syncthreads();
-
-            if (own_out) {
-                const float4 xl = lds[b][ty + kR][tx];
-                const float4 xr = lds[b][ty + kR][tx + 2];
-                const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
-                                     ctr4.z, ctr4.w, xr.x,   xr.y,   xr.z,   xr.w};
-                float4 ym[kR], yp[kR];
+                // Refill the queue (plane zz+4) and prefetch plane zz+1's halo
+                // and streams; all in bounds since zz+4 <= nz-1.
+                q[(j + 8) % 9] = cur_at(zz + kR, own_off, own_in);
+                float4 h_nxt[kHaloPer];
 #pragma unroll
-                for (int r = 1; r <= kR; ++r) {
-                    ym[r - 1] = lds[b][ty + kR - r][tx + 1];
-                    yp[r - 1] = lds[b][ty + kR + r][tx + 1];
+                for (int k = 0; k < kHaloPer; ++k) h_nxt[k] = cur_at(zz + 1, h_off[k], h_in[k]);
+                float4 pv_nxt = make_float4(0.f, 0.f, 0.f, 0.f), vv_nxt = pv_nxt;
+                if constexpr (!kHeat) {
+                    pv_nxt = stream_at(a.prev, zz + 1);
+                    vv_nxt = stream_at(a.vel2, zz + 1);
                 }
-                float4 out;
+
+                const int b = (zz - z_begin) & 1;
+                const float4 ctr4 = q[(j + 4) % 9];
+                lds[b][ty + kR][tx + 1] = ctr4;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float p = w[4 + i];
-                    float res;
-                    if constexpr (kExact) {
-                        // src/kernels/stencil_naive.cu:75-88, op for op.
-                        float lx = Fd<float>::c0 * p * a.s.inv_dx2;
-                        float ly = Fd<float>::c0 * p * a.s.inv_dy2;
-                        float lz = Fd<float>::c0 * p * a.s.inv_dz2;
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h_cur[k];
+                __syncthreads();
+
+                if (own_out) {
+                    const float4 xl = lds[b][ty + kR][tx];
+                    const float4 xr = lds[b][ty + kR][tx + 2];
+                    const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
+                                         ctr4.z, ctr4.w, xr.x,   xr.y,   xr.z,   xr.w};
+                    float4 ym[kR], yp[kR];
 #pragma unroll
-                        for (int r = 1; r <= kR; ++r) {
-                            lx += Fd<float>::c(r) * (w[4 + i + r] + w[4 + i - r]) * a.s.inv_dx2;
-                            ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) *
-                                  a.s.inv_dy2;
-                            lz += Fd<float>::c(r) *
-                                  (comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i)) *
-                                  a.s.inv_dz2;
-                        }
-                        if constexpr (kHeat) {
-                            res = p + a.coef * (lx + ly + lz);
-                        } else {
-                            res = 2.f * p - comp(pv_cur, i) +
-                                  a.s.dt2 * comp(vv_cur, i) * (lx + ly + lz);
-                        }
-                    } else {
-                        // Combined weights w_r = c_r / h^2, one accumulator.
-                        float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
-#pragma unroll
-                        for (int r = 1; r <= kR; ++r) {
-                            acc = __builtin_fmaf(a.wx[r - 1], w[4 + i + r] + w[4 + i - r], acc);
-                            acc = __builtin_fmaf(a.wy[r - 1],
-                                                 comp(yp[r - 1], i) + comp(ym[r - 1], i), acc);
-                            acc = __builtin_fmaf(
-                                a.wz[r - 1],
-                                comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i), acc);
-                        }
-                        if constexpr (kHeat) {
-                            res = acc;
-                        } else {
-                            const float base = __builtin_fmaf(2.f, p, -comp(pv_cur, i));
-                            res = __builtin_fmaf(a.s.dt2 * comp(vv_cur, i), acc, base);
-                        }
+                    for (int r = 1; r <= kR; ++r) {
+                        ym[r - 1] = lds[b][ty + kR - r][tx + 1];
+                        yp[r - 1] = lds[b][ty + kR + r][tx + 1];
                     }
-                    dev::set_comp(out, i, res);
+                    float4 out;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float p = w[4 + i];
+                        float res;
+                        if constexpr (kExact) {
+                            // src/kernels/stencil_naive.cu:75-88, op for op.
+                            float lx = Fd<float>::c0 * p * a.s.inv_dx2;
+                            float ly = Fd<float>::c0 * p * a.s.inv_dy2;
+                            float lz = Fd<float>::c0 * p * a.s.inv_dz2;
+#pragma unroll
+                            for (int r = 1; r <= kR; ++r) {
+                                lx += Fd<float>::c(r) * (w[4 + i + r] + w[4 + i - r]) * a.s.inv_dx2;
+                                ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) *
+                                      a.s.inv_dy2;
+                                lz += Fd<float>::c(r) *
+                                      (comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i)) *
+                                      a.s.inv_dz2;
+                            }
+                            if constexpr (kHeat) {
+                                res = p + a.coef * (lx + ly + lz);
+                            } else {
+                                res = 2.f * p - comp(pv_cur, i) +
+                                      a.s.dt2 * comp(vv_cur, i) * (lx + ly + lz);
+                            }
+                        } else {
+                            // Combined weights w_r = c_r / h^2, one accumulator.
+                            float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
+#pragma unroll
+                            for (int r = 1; r <= kR; ++r) {
+                                acc = __builtin_fmaf(a.wx[r - 1], w[4 + i + r] + w[4 + i - r], acc);
+                                acc = __builtin_fmaf(a.wy[r - 1],
+                                                     comp(yp[r - 1], i) + comp(ym[r - 1], i), acc);
+                                acc = __builtin_fmaf(
+                                    a.wz[r - 1],
+                                    comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i), acc);
+                            }
+                            if constexpr (kHeat) {
+                                res = acc;
+                            } else {
+                                const float base = __builtin_fmaf(2.f, p, -comp(pv_cur, i));
+                                res = __builtin_fmaf(a.s.dt2 * comp(vv_cur, i), acc, base);
+                            }
+                        }
+                        dev::set_comp(out, i, res);
+                    }
+                    st_stream<NT>(a.next + static_cast<size_t>(zz) * plane + own_off, out);
                 }
-                float* nb = a.next + static_cast<size_t>(zz) * plane;
-                dev::store_stream(reinterpret_cast<float4*>(nb + own_off), out);
-            }
-            h_cur = h_nxt;
-            pv_cur = pv_nxt;
-            vv_cur = vv_nxt;
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k) h_cur[k] = h_nxt[k];
+                pv_cur = pv_nxt;
+                vv_cur = vv_nxt;
             }
         }
     }
@@ -223,22 +247,29 @@ __global__ __launch_bounds__(16 * TY) void zq_kernel(ZqArgs a) {
 
 using KernelFn = void (*)(ZqArgs);
 
-template <int MODE>
-KernelFn pick_kernel(int ty) {
-    switch (ty) {
-        case 16: return zq_kernel<16, MODE>;
-        case 24: return zq_kernel<24, MODE>;
-        case 32: return zq_kernel<32, MODE>;
-        default: return nullptr;
-    }
+// Tile shapes (float4 columns x rows) the library is built with.
+struct TileShape {
+    int txv, ty;
+};
+constexpr TileShape kShapes[] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}};
+
+template <int MODE, bool NT>
+KernelFn pick_shape(int txv, int ty) {
+    if (txv == 16 && ty == 16) return zq_kernel<16, 16, MODE, NT>;
+    if (txv == 16 && ty == 32) return zq_kernel<16, 32, MODE, NT>;
+    if (txv == 32 && ty == 8) return zq_kernel<32, 8, MODE, NT>;
+    if (txv == 32 && ty == 16) return zq_kernel<32, 16, MODE, NT>;
+    if (txv == 64 && ty == 4) return zq_kernel<64, 4, MODE, NT>;
+    if (txv == 64 && ty == 8) return zq_kernel<64, 8, MODE, NT>;
+    return nullptr;
 }
 
-KernelFn pick(int ty, int mode) {
+KernelFn pick(int txv, int ty, int mode, bool nt) {
     switch (mode) {
-        case kWaveFused: return pick_kernel<kWaveFused>(ty);
-        case kWaveExact: return pick_kernel<kWaveExact>(ty);
-        case kHeatFused: return pick_kernel<kHeatFused>(ty);
-        default: return pick_kernel<kHeatExact>(ty);
+        case kWaveFused: return nt ? pick_shape<kWaveFused, true>(txv, ty) : pick_shape<kWaveFused, false>(txv, ty);
+        case kWaveExact: return nt ? pick_shape<kWaveExact, true>(txv, ty) : pick_shape<kWaveExact, false>(txv, ty);
+        case kHeatFused: return nt ? pick_shape<kHeatFused, true>(txv, ty) : pick_shape<kHeatFused, false>(txv, ty);
+        default: return nt ? pick_shape<kHeatExact, true>(txv, ty) : pick_shape<kHeatExact, false>(txv, ty);
     }
 }
 
@@ -258,19 +289,19 @@ void fill_weights(ZqArgs& a, double scale) {
 hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
     shape = zq_resolve(d, shape);
-    KernelFn fn = pick(shape.ty, mode);
+    KernelFn fn = pick(shape.txv, shape.ty, mode, shape.nt != 0);
     if (!fn) return hipErrorInvalidValue;
     const int64_t inz = d.nz - 2 * kR;
     a.nx = static_cast<int>(d.nx);
     a.ny = static_cast<int>(d.ny);
     a.nz = static_cast<int>(d.nz);
-    a.tiles_x = static_cast<int>((d.nx - 2 * kR + 63) / 64);
+    a.tiles_x = static_cast<int>((d.nx + 4 * shape.txv - 1) / (4 * shape.txv));
     a.tiles_y = static_cast<int>((d.ny - 2 * kR + shape.ty - 1) / shape.ty);
     a.chunk = static_cast<int>((inz + shape.n_chunks - 1) / shape.n_chunks);
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = (shape.xcd_remap != 0 && nwg % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg)), dim3(16, shape.ty, 1), 0, st, a);
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg)), dim3(shape.txv * shape.ty), 0, st, a);
     return hipGetLastError();
 }
 
@@ -285,14 +316,36 @@ bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs) {
     return true;
 }
 
+bool zq_shape_valid(int txv, int ty) {
+    for (const auto& s : kShapes)
+        if (s.txv == txv && s.ty == ty) return true;
+    return false;
+}
+
 ZqShape zq_resolve(const Dims& d, ZqShape s) {
-    if (s.ty != 16 && s.ty != 24 && s.ty != 32) s.ty = 16;
+    if (!zq_shape_valid(s.txv, s.ty)) {
+        // Widest tile whose row padding past nx is smallest (ties go wide):
+        // 256-cell rows at nx = 512, 128 at nx = 128, ...; 512 threads each.
+        int best = 16;
+        int64_t best_waste = -1;
+        for (int t : {64, 32, 16}) {
+            const int64_t w = ((d.nx + 4 * t - 1) / (4 * t)) * (4 * t) - d.nx;
+            if (best_waste < 0 || w < best_waste) {
+                best = t;
+                best_waste = w;
+            }
+        }
+        s.txv = best;
+        s.ty = best == 64 ? 8 : best == 32 ? 16 : 32;
+    }
+    if (s.nt < 0) s.nt = 1;
     const int64_t inz = d.nz - 2 * kR;
     if (s.n_chunks <= 0) {
-        // Aim for >= 1024 workgroups (4 per CU on 256 CUs) with chunks of at
-        // least 32 planes, rounded so the total is a multiple of 8 (XCDs).
-        const int64_t tiles = ((d.nx - 2 * kR + 63) / 64) * ((d.ny - 2 * kR + s.ty - 1) / s.ty);
-        int64_t want = (1024 + tiles - 1) / tiles;
+        // About two workgroups per CU (256 CUs), chunks of at least 32 planes:
+        // long z-marches stream best.
+        const int64_t tiles =
+            ((d.nx + 4 * s.txv - 1) / (4 * s.txv)) * ((d.ny - 2 * kR + s.ty - 1) / s.ty);
+        int64_t want = (512 + tiles / 2) / tiles;
         const int64_t max_chunks = inz >= 32 ? inz / 32 : 1;
         if (want > max_chunks) want = max_chunks;
         if (want < 1) want = 1;

@@ -55,12 +55,14 @@ def test_invalid_arguments_rejected_without_device():
     g0 = stencil_grid(0, 32, 32, 10.0, 10.0, 10.0, 1e-3)
     assert L.stencil_launch_naive_f64(a, a, a, b, C.byref(g0), null) == 1
     assert L.stencil_heat_step_f32(a, a, 1.0, C.byref(g), 0, null) == 1
-    assert L.stencil_launch_tiled_ex_f32(a, a, a, b, C.byref(g), 0, 7, 0, -1, null) in (1, 3)
+    bad_shape = ss.tile_shape(16, 7)
+    assert L.stencil_launch_tiled_ex_f32(a, a, a, b, C.byref(g), 0, C.byref(bad_shape), null) == 1
+    ok_shape = ss.tile_shape(16, 32)
     # unaligned base -> the tiled kernel declines (the public launcher then runs naive)
     u = C.c_void_p(0x1004)
-    assert L.stencil_launch_tiled_ex_f32(u, a, a, b, C.byref(g), 0, 16, 0, -1, null) == 3
+    assert L.stencil_launch_tiled_ex_f32(u, a, a, b, C.byref(g), 0, C.byref(ok_shape), null) == 3
     g_odd = stencil_grid(30, 32, 32, 10.0, 10.0, 10.0, 1e-3)
-    assert L.stencil_launch_tiled_ex_f32(a, a, a, b, C.byref(g_odd), 0, 16, 0, -1, null) == 3
+    assert L.stencil_launch_tiled_ex_f32(a, a, a, b, C.byref(g_odd), 0, None, null) == 3
     assert L.stencil_strerror(1) == b"invalid argument"
     from stencil_solver_amd._lib import stencil_config
     cfg = stencil_config()

@@ -1,12 +1,11 @@
 """Parity of the gfx950 kernels with the oracle and the reference's golden vectors.
 
 Bars (tolerances written in each test):
-* naive kernel, FP32 and FP64, and the tiled kernel in its exact-association
-  mode: bit-identical to the oracle / to the reference CPU path built without
-  FMA contraction.
-* tiled kernel in its default FMA-fused form: relative L2 <= 1e-5 against the
-  oracle (the north-star FP32 tolerance), and against the reference's own
-  Release build.
+* naive and tiled kernels (FP32 and FP64): bit-identical to the oracle / to the
+  reference CPU path built without FMA contraction, and within 1e-5 relative
+  L2 of the reference's own Release (FMA-contracted) build.
+* the opt-in FMA-fused tiled form: relative L2 <= 1e-5 after 5 steps, <= 1e-4
+  after 200 (its rounding drifts from the reference association).
 * at the full 512^3 / 1024^3 sizes (too large for the oracle): size-independent
   properties -- exact-mode tiled == naive bit for bit, fused within 1e-5 of
   naive, halos untouched, mirror symmetry of a centred pulse.
@@ -86,11 +85,7 @@ def test_wave_solver_matches_reference_golden(ss, dev, variant, steps):
         s.step()
     out = s.download()
     ref = gd[f"naive{steps}_nofma"]
-    if variant == "naive":
-        assert np.array_equal(out, ref)
-    else:
-        rel = np.linalg.norm((out - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64))
-        assert rel <= 1e-5, rel
+    assert np.array_equal(out, ref)
     nat = gd[f"naive{steps}_native"].astype(np.float64)
     assert np.linalg.norm(out - nat) / np.linalg.norm(nat) <= 1e-5
 
@@ -113,11 +108,9 @@ def test_heat_solver_matches_reference_golden(ss, dev, variant):
     for _ in range(20):
         s.step()
     out = s.download()
-    ref = gd["heat20_nofma"]
-    if variant == "naive":
-        assert np.array_equal(out, ref)
-    else:
-        assert np.linalg.norm((out - ref).astype(np.float64)) / np.linalg.norm(ref.astype(np.float64)) <= 1e-5
+    assert np.array_equal(out, gd["heat20_nofma"])
+    nat = gd["heat20_native"].astype(np.float64)
+    assert np.linalg.norm(out - nat) / np.linalg.norm(nat) <= 1e-5
 
 
 @pytest.mark.parametrize("variant", ["naive", "tiled"])
@@ -129,8 +122,7 @@ def test_acoustic128_matches_reference_golden(ss, dev, variant, golden_meta):
     vel2 = np.full_like(p0, np.float32(2000.0) * np.float32(2000.0))
     grid = ss.GridDims.cube(128, np.float32(10.0), np.float32(1e-3))
     out = ring_run(ss, variant, p0, vel2, grid, 200, dev)
-    if variant == "naive":
-        assert sha(out) == golden_meta["acoustic128_nofma"]["sha_final"]
+    assert sha(out) == golden_meta["acoustic128_nofma"]["sha_final"]
     mid = gd["mid_z_nofma"].astype(np.float64)
     assert np.linalg.norm(out[64] - mid) / np.linalg.norm(mid) <= 1e-5
     nat = gd["mid_z_native"].astype(np.float64)
@@ -168,16 +160,17 @@ def test_naive_bit_exact_vs_oracle(ss, dev, oracle, shape):
 
 
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[2] % 4 == 0])
-@pytest.mark.parametrize("ty,n_chunks,xcd", [(16, 0, -1), (24, 1, 0), (32, 3, 1), (16, 5, 0)])
-def test_tiled_exact_bit_exact_vs_oracle(ss, dev, oracle, shape, ty, n_chunks, xcd):
+@pytest.mark.parametrize("tx4,ty,n_chunks,xcd,nt", [(16, 16, 0, -1, -1), (16, 32, 1, 0, 1), (32, 8, 3, 1, 0),
+                                                   (32, 16, 5, 0, 1), (64, 4, 2, 1, 0), (64, 8, 0, -1, 1)])
+def test_tiled_exact_bit_exact_vs_oracle(ss, dev, oracle, shape, tx4, ty, n_chunks, xcd, nt):
     p0, vel2 = _inputs(oracle, shape, 12)
     grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
                        np.float32(8e-4))
     ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 5, grid.dy, grid.dz)
-    out = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=True, ty=ty, n_chunks=n_chunks, xcd_remap=xcd)
+    kw = dict(tx4=tx4, ty=ty, n_chunks=n_chunks, xcd_remap=xcd, nontemporal=nt)
+    out = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=True, **kw)
     assert np.array_equal(out, ref)
-    fused = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=False, ty=ty, n_chunks=n_chunks,
-                     xcd_remap=xcd)
+    fused = ring_run(ss, "tiled_ex", p0, vel2, grid, 5, dev, exact=False, **kw)
     assert oracle.rel_l2_interior(fused, ref) <= 1e-5
 
 
@@ -188,10 +181,7 @@ def test_tiled_default_vs_oracle(ss, dev, oracle, shape):
     grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(10.0), np.float32(10.0),
                        np.float32(1e-3))
     ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 8)
-    out = ring_run(ss, "tiled", p0, vel2, grid, 8, dev)
-    assert oracle.rel_l2_interior(out, ref) <= 1e-5
-    if shape[2] % 4 != 0:
-        assert np.array_equal(out, ref)
+    assert np.array_equal(ring_run(ss, "tiled", p0, vel2, grid, 8, dev), ref)
 
 
 def test_layered_model_128_50_steps(ss, dev, oracle):
@@ -203,10 +193,23 @@ def test_layered_model_128_50_steps(ss, dev, oracle):
     grid = ss.GridDims.cube(128, np.float32(10.0), np.float32(1e-3))
     ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 50)
     assert np.array_equal(ring_run(ss, "naive", p0, vel2, grid, 50, dev), ref)
-    assert oracle.rel_l2_interior(ring_run(ss, "tiled", p0, vel2, grid, 50, dev), ref) <= 1e-5
+    assert np.array_equal(ring_run(ss, "tiled", p0, vel2, grid, 50, dev), ref)
 
 
-@pytest.mark.parametrize("exact", [True, False])
+def test_fused_form_drift_bound(ss, dev, golden_meta):
+    """The opt-in FMA form after 200 steps of the 128^3 workload: <= 1e-4."""
+    gd = load_golden("acoustic_128.npz")
+    p0 = ss.host_gaussian((128, 128, 128), 0.5, 0.5, 0.5, 3.0)
+    vel2 = np.full_like(p0, np.float32(2000.0) * np.float32(2000.0))
+    grid = ss.GridDims.cube(128, np.float32(10.0), np.float32(1e-3))
+    out = ring_run(ss, "tiled_ex", p0, vel2, grid, 200, dev, exact=False)
+    mid = gd["mid_z_nofma"].astype(np.float64)
+    rel = np.linalg.norm(out[64] - mid) / np.linalg.norm(mid)
+    print(f"fused-form rel-L2 after 200 steps: {rel:.3e}")
+    assert rel <= 1e-4
+
+
+@pytest.mark.parametrize("exact", [None, True, False])
 def test_heat_tiled_vs_oracle(ss, dev, oracle, exact):
     import torch
 
@@ -220,10 +223,10 @@ def test_heat_tiled_vs_oracle(ss, dev, oracle, exact):
         ss.heat_step(a, b, 1.0, grid, exact=exact)
         a, b = b, a
     out = a.cpu().numpy()
-    if exact:
-        assert np.array_equal(out, ref)
-    else:
+    if exact is False:
         assert oracle.rel_l2_interior(out, ref) <= 1e-5
+    else:
+        assert np.array_equal(out, ref)
     a, b = to_dev(t0, dev), torch.zeros(shape, device=dev)
     for _ in range(10):
         ss.heat_step(a, b, 1.0, grid, variant="naive")
@@ -319,3 +322,33 @@ def test_fullsize_tiled_equals_naive(ss, dev, n):
     w = bufs[2]
     assert torch.all(w[:R] == 3.25) and torch.all(w[:, :, -R:] == 3.25) and torch.all(w[:, :R] == 3.25)
     assert torch.equal(a, torch.flip(a, dims=[2]))  # constant-in-x medium, centred pulse
+
+
+# ── autotuner ─────────────────────────────────────────────────────────────────
+def test_autotuner_picks_and_caches_shape(ss, dev, oracle, tmp_path):
+    import torch
+
+    shape = (40, 44, 136)
+    p0, vel2 = _inputs(oracle, shape, 21)
+    grid = ss.GridDims(136, 44, 40, np.float32(10.0), np.float32(10.0), np.float32(10.0), np.float32(1e-3))
+    ss.clear_tuning()
+    before, hit = ss.tuned_shape(grid)
+    assert not hit
+    scratch = torch.zeros(shape, device=dev)
+    csv = tmp_path / "sweep.csv"
+    best = ss.tune_tiled(to_dev(p0, dev), to_dev(p0, dev), to_dev(vel2, dev), scratch, grid, reps=2,
+                         csv_path=csv)
+    after, hit = ss.tuned_shape(grid)
+    assert hit and after == best
+    lines = csv.read_text().strip().splitlines()
+    assert lines[0].startswith("nx,ny,nz,tile_x,tile_y,n_chunks,nontemporal,naive_ms,tiled_ms,speedup")
+    assert len(lines) > 20
+    # the tuned launch is still bit-identical to the oracle
+    ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 6)
+    assert np.array_equal(ring_run(ss, "tiled", p0, vel2, grid, 6, dev), ref)
+    # heat tuner on the same dims
+    t0 = oracle.heat_noise_cube(shape, side=8, hot=10.0, seed=2)
+    hb = ss.tune_heat(to_dev(t0, dev), torch.zeros(shape, device=dev), grid, reps=2)
+    assert ss.tuned_shape(grid, "heat") == (hb, True)
+    ss.clear_tuning()
+    assert not ss.tuned_shape(grid)[1]
