@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="skip the side measurements")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage / FP64 study")
+    ap.add_argument("--precision-n", type=int, nargs="*", default=None,
+                    help="grid sizes of the 16-bit storage study (default: --n)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 2 STREAM copies of one field (known bytes) for counter scaling")
     ap.add_argument("--tune-csv", default=None, help="write the autotuner sweep to this CSV")
@@ -213,6 +216,29 @@ def main():
         result["extras"] = extras
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
+    if not args.no_precision:
+        sizes = args.precision_n if args.precision_n else [n]
+        result["precision"] = {}
+        for pn in sizes:
+            if pn == n:
+                pp0, pv2 = p0, vel2
+            else:
+                pshape = (pn, pn, pn)
+                pp0 = torch.from_numpy(ss.host_gaussian(pshape, 0.5, 0.5, 0.5, 3.0)).to(dev)
+                pv2 = torch.from_numpy(ss.host_layered_noise_vel2(pshape, 16, 1500.0, 4000.0, 0.05, 2)).to(dev)
+            result["precision"][str(pn)] = precision_study(ss, pp0, pv2, pn, args.steps)
+            del pp0, pv2
+        torch.cuda.empty_cache()
+        p512 = result["precision"].get(str(n))
+        if p512:
+            ok = {k: v for k, v in p512.items() if k in ("bf16", "fp16", "bfp16") and v["rel_l2_vs_fp64"] <= 1e-3}
+            best = max(ok, key=lambda k: ok[k]["gpts"]) if ok else None
+            result["bf16_store"] = {
+                "bf16_gpts": p512["bf16"]["gpts"], "bf16_rel_l2_vs_fp64": p512["bf16"]["rel_l2_vs_fp64"],
+                "best_16bit_within_1e-3": best,
+                "best_16bit_gpts": ok[best]["gpts"] if best else None,
+                "speedup_vs_fp32": round(ok[best]["gpts"] / gpts, 3) if best else None}
+
     if args.calibrate:
         # Known-size copy: n^3 floats read + n^3 floats written per launch, the
         # reference the FETCH_SIZE / WRITE_SIZE counters are scaled against.
@@ -229,6 +255,82 @@ def main():
         result["cpu_baseline"] = cpu_baseline(p0_h, vel2_h, h, dt, interior, args.cpu_seconds)
 
     print(json.dumps(result), flush=True)
+
+
+def precision_study(ss, p0, vel2, n, steps):
+    """16-bit field storage against an FP64 run of the same seeded inputs:
+    `steps` steps of the acoustic workload (h = 10 m, dt = 1e-3 s); for FP32 and
+    each 16-bit format: Gpts/s, relative L2 and max-abs error of the final p
+    against FP64 over the interior, and the drift of the leapfrog energy
+    (FP64 reductions) between the first and the last step."""
+    import numpy as np
+    import torch
+
+    dev = p0.device
+    shape = (n, n, n)
+    interior = (n - 8) ** 3
+    grid = ss.GridDims.cube(n, np.float32(10.0), np.float32(1e-3))
+    red = ss.Reducer(dev)
+    out = {}
+
+    def energy(a, b, v):
+        k, p = red.energy(a, b, v, grid)
+        return k + p
+
+    def ring(step_fn, init, zero, decode=lambda x: x, v_for_energy=None):
+        b = [init(), init(), zero()]
+        i = 0
+        step_fn(b[(i + 1) % 3], b[i], b[(i + 2) % 3])
+        i = 1
+        e_first = energy(decode(b[0]), decode(b[1]), v_for_energy) if v_for_energy is not None else None
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps - 1):
+            step_fn(b[(i + 1) % 3], b[i], b[(i + 2) % 3])
+            i = (i + 1) % 3
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / max(steps - 1, 1)
+        last, prev = b[(i + 1) % 3], b[i]
+        e_last = energy(decode(prev), decode(last), v_for_energy) if v_for_energy is not None else None
+        return last, ms, e_first, e_last
+
+    # FP64 reference (naive kernel, exact association), same inputs widened.
+    v64 = vel2.double()
+    ref, ms64, e0, e1 = ring(lambda c, p, o: ss.launch_naive(c, p, v64, o, grid),
+                             lambda: p0.double(), lambda: torch.zeros(shape, dtype=torch.float64, device=dev),
+                             v_for_energy=v64)
+    out["fp64_ref"] = {"ms": round(ms64, 4), "energy_drift": (e1 - e0) / e0}
+
+    def report(name, last32, ms, ea, eb, bytes_pt):
+        diff = (last32.double() - ref)[4:-4, 4:-4, 4:-4]
+        out[name] = {"gpts": round(interior / (ms * 1e-3) / 1e9, 2), "ms": round(ms, 5),
+                     "bytes_per_point": bytes_pt,
+                     "rel_l2_vs_fp64": red.rel_l2(last32, ref, grid),
+                     "max_abs_vs_fp64": float(diff.abs().max()),
+                     "energy_drift": (eb - ea) / ea}
+        del diff
+
+    last, ms, ea, eb = ring(lambda c, p, o: ss.launch_tiled(c, p, vel2, o, grid), lambda: p0.clone(),
+                            lambda: torch.zeros_like(p0), v_for_energy=vel2)
+    report("fp32", last, ms, ea, eb, 16.0)
+    del last
+    for fmt in ("bf16", "fp16", "bfp16"):
+        vs = ss.pow2_scale_for(float(vel2.max())) if fmt == "fp16" else 1.0
+        gv = ss.encode16(fmt, vel2, ss.Field16(fmt, shape, dev, vs))
+        vdec = ss.decode16(gv)
+        last, ms, ea, eb = ring(lambda c, p, o: ss.launch_tiled_16(c, p, gv, o, grid),
+                                lambda: ss.encode16(fmt, p0, ss.Field16(fmt, shape, dev, 1.0)),
+                                lambda: ss.encode16(fmt, torch.zeros_like(p0), ss.Field16(fmt, shape, dev, 1.0)),
+                                decode=ss.decode16, v_for_energy=vdec)
+        report(fmt, ss.decode16(last), ms, ea, eb, 8.25 if fmt == "bfp16" else 8.0)
+        del gv, vdec, last
+    del ref, v64
+    print(f"[bench] precision {n}^3 x {steps}: " +
+          ", ".join(f"{k}: {v.get('gpts', '-')} Gpts/s rel {v.get('rel_l2_vs_fp64', 0):.2e}" for k, v in out.items()
+                    if k != "fp64_ref"), file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(p0_h, vel2_h, h, dt, interior, seconds):

@@ -155,6 +155,61 @@ void stencil_clear_tuning(void);
  * array, n % 4 == 0, 16-byte aligned. */
 int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* hip_stream);
 
+/* ---- 16-bit field storage (config "precision"): p, p_prev and vel2 stored in
+ * 16 bits, arithmetic in FP32 with the reference association.  Decoding is
+ * exact, so each format is bit-reproducible against a CPU restatement.
+ *   STENCIL_STORE_BF16  bfloat16, round to nearest even
+ *   STENCIL_STORE_FP16  IEEE half of value*scale, scale a power of two per field
+ *   STENCIL_STORE_BFP16 block floating point: int16 mantissas + one int8
+ *                       exponent per 16 x-consecutive values (v = m*2^(e-15),
+ *                       e = frexp exponent of the block max); needs the
+ *                       zero halo of Dirichlet walls, and nx % 16 == 0. ---- */
+enum stencil_storage {
+    STENCIL_STORE_BF16 = 1,
+    STENCIL_STORE_FP16 = 2,
+    STENCIL_STORE_BFP16 = 3
+};
+typedef struct stencil_field16 {
+    void* data;  /* nx*ny*nz uint16 */
+    int8_t* exp; /* nx*ny*nz/16 block exponents (BFP16 only, else NULL) */
+} stencil_field16;
+
+/* Bytes of the data and exponent arrays of one field of this grid. */
+int stencil_field16_bytes(int storage, const stencil_grid* grid, uint64_t* data_bytes,
+                          uint64_t* exp_bytes);
+/* n values (n % 16 == 0) between FP32 and 16-bit storage; scale is the
+ * power-of-two field scale (FP16 only, 1 otherwise). */
+int stencil_encode16(int storage, const float* src, const stencil_field16* dst, uint64_t n,
+                     float scale, void* hip_stream);
+int stencil_decode16(int storage, const stencil_field16* src, float* dst, uint64_t n, float scale,
+                     void* hip_stream);
+/* One acoustic step on 16-bit fields with the tiled kernel.  p_scale is the
+ * scale of p_cur / p_prev / p_next, v_scale of vel2 (both 1 unless FP16). */
+int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
+                            const stencil_field16* vel2, const stencil_field16* p_next,
+                            const stencil_grid* grid, float p_scale, float v_scale,
+                            const stencil_tile_shape* tile, void* hip_stream);
+
+/* ---- FP64 reductions over the interior (deterministic: fixed-order two-pass
+ * reduction through `work`, stencil_reduce_workspace_bytes() of device
+ * memory; results land in out[0..2] on the device). ---- */
+uint64_t stencil_reduce_workspace_bytes(void);
+/* Leapfrog energy of the pair (p^n, p^{n+1}): out[0] = sum (p^{n+1}-p^n)^2 /
+ * (vel2 dt^2), out[1] = -sum p^{n+1} * Lap(p^n); out[0] + out[1] is conserved
+ * by the exact scheme. */
+int stencil_energy_f32(const float* p_n, const float* p_np1, const float* vel2,
+                       const stencil_grid* grid, double* out, double* work, void* hip_stream);
+int stencil_energy_f64(const double* p_n, const double* p_np1, const double* vel2,
+                       const stencil_grid* grid, double* out, double* work, void* hip_stream);
+/* out[0] = sum (a-b)^2, out[1] = sum b^2 (relative L2 = sqrt(out0/out1)). */
+int stencil_diff_norms_f32_f64(const float* a, const double* b, const stencil_grid* grid,
+                               double* out, double* work, void* hip_stream);
+/* out[0] = sum a*b. */
+int stencil_dot_f64(const double* a, const double* b, const stencil_grid* grid, double* out,
+                    double* work, void* hip_stream);
+int stencil_dot_f32(const float* a, const float* b, const stencil_grid* grid, double* out,
+                    double* work, void* hip_stream);
+
 /* Seeded host-side input generators (OpenMP), the same values on every
  * platform: see stencil::init_gaussian / init_layered_noise_vel2 /
  * init_noise_cube in include/stencil/grid.hpp. */

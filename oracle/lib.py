@@ -64,6 +64,17 @@ class Oracle:
         L.oracle_rel_l2_interior_f32.restype = _D
         L.oracle_rel_l2_interior_f64f32.argtypes = [_P, _P] + [_SZ] * 3
         L.oracle_rel_l2_interior_f64f32.restype = _D
+        L.oracle_f32_to_bf16.argtypes = [_F]
+        L.oracle_f32_to_bf16.restype = C.c_uint16
+        L.oracle_bf16_to_f32.argtypes = [C.c_uint16]
+        L.oracle_bf16_to_f32.restype = _F
+        L.oracle_f32_to_fp16.argtypes = [_F]
+        L.oracle_f32_to_fp16.restype = C.c_uint16
+        L.oracle_fp16_to_f32.argtypes = [C.c_uint16]
+        L.oracle_fp16_to_f32.restype = _F
+        L.oracle_encode16.argtypes = [C.c_int, _P, _P, _P, _SZ, _F]
+        L.oracle_decode16.argtypes = [C.c_int, _P, _P, _P, _SZ, _F]
+        L.oracle_wave_step16.argtypes = [C.c_int] + [_P] * 8 + [_SZ] * 3 + [_F] * 6
 
     # -- coefficients ------------------------------------------------------
     def fd_coeffs(self, dtype=np.float32) -> np.ndarray:
@@ -146,6 +157,37 @@ class Oracle:
         if b.dtype == np.float64:
             return float(self.lib.oracle_rel_l2_interior_f64f32(_ptr(a), _ptr(b), *_dims(a)))
         return float(self.lib.oracle_rel_l2_interior_f32(_ptr(a), _ptr(b), *_dims(a)))
+
+    # -- 16-bit storage ----------------------------------------------------
+    FORMATS = {"bf16": 1, "fp16": 2, "bfp16": 3}
+
+    def encode16(self, fmt, x, scale=1.0):
+        """float32 array -> (uint16 array, int8 block exponents or None)."""
+        f = self.FORMATS[fmt]
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        d = np.zeros(x.shape, dtype=np.uint16)
+        e = np.zeros(x.size // 16, dtype=np.int8)
+        self.lib.oracle_encode16(f, _ptr(x), _ptr(d), _ptr(e), x.size, scale)
+        return d, (e if f == 3 else None)
+
+    def decode16(self, fmt, d, e=None, scale=1.0):
+        f = self.FORMATS[fmt]
+        out = np.empty(d.shape, dtype=np.float32)
+        ee = e if e is not None else np.zeros(max(d.size // 16, 1), dtype=np.int8)
+        self.lib.oracle_decode16(f, _ptr(d), _ptr(ee), _ptr(out), d.size, scale)
+        return out
+
+    def wave_step16(self, fmt, cur, prev, vel2, nxt, dx, dt, p_scale=1.0, v_scale=1.0, dy=None, dz=None):
+        """One 16-bit step; each field is (uint16 array, int8 exps or None); nxt is updated."""
+        dy = dx if dy is None else dy
+        dz = dx if dz is None else dz
+        f = self.FORMATS[fmt]
+        dummy = np.zeros(max(cur[0].size // 16, 1), dtype=np.int8)
+        ex = lambda fe: _ptr(fe[1]) if fe[1] is not None else _ptr(dummy)
+        nz, ny, nx = cur[0].shape
+        self.lib.oracle_wave_step16(f, _ptr(cur[0]), ex(cur), _ptr(prev[0]), ex(prev), _ptr(vel2[0]), ex(vel2),
+                                    _ptr(nxt[0]), ex(nxt), nx, ny, nz, dx, dy, dz, dt, p_scale, v_scale)
+        return nxt
 
     # -- solver runs -------------------------------------------------------
     def wave_run(self, p0, vel2, dx, dt, n_steps, dy=None, dz=None):

@@ -14,6 +14,7 @@
 #include "stencil_oracle.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define R ORACLE_R
@@ -304,4 +305,150 @@ double oracle_rel_l2_interior_f64f32(const float* a, const double* b, size_t nx,
                 den += b[i] * b[i];
             }
     return den > 0.0 ? sqrt(num / den) : sqrt(num);
+}
+
+/* ---- 16-bit field storage ---- */
+
+static uint32_t f2u(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+static float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+uint16_t oracle_f32_to_bf16(float f) {
+    const uint32_t b = f2u(f);
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+float oracle_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+/* IEEE 754 binary32 -> binary16, round to nearest even, subnormals kept,
+ * overflow to infinity (IEEE 754-2008 section 4.3.1 / 7.4). */
+uint16_t oracle_f32_to_fp16(float f) {
+    const uint32_t x = f2u(f);
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const int32_t e = (int32_t)((x >> 23) & 0xFF);
+    uint32_t m = x & 0x7FFFFFu;
+    if (e == 0xFF) return sign | 0x7C00u | (m ? 0x200u : 0u); /* inf / nan */
+    int32_t he = e - 127 + 15;
+    if (he >= 0x1F) return sign | 0x7C00u; /* overflow */
+    if (he <= 0) {
+        /* subnormal half (or zero): value = m' * 2^-24 */
+        if (he < -10) return sign; /* rounds to zero */
+        m |= 0x800000u;            /* implicit bit */
+        const int shift = 14 - he; /* 14..24 */
+        uint32_t hm = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (hm & 1u))) hm += 1;
+        return sign | (uint16_t)hm; /* a carry into the exponent field is correct */
+    }
+    uint32_t hm = m >> 13;
+    const uint32_t rem = m & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u))) {
+        hm += 1;
+        if (hm == 0x400u) {
+            hm = 0;
+            he += 1;
+            if (he >= 0x1F) return sign | 0x7C00u;
+        }
+    }
+    return sign | (uint16_t)((uint32_t)he << 10) | (uint16_t)hm;
+}
+
+float oracle_fp16_to_f32(uint16_t h) {
+    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1F;
+    const uint32_t m = h & 0x3FFu;
+    if (e == 0) {
+        const float v = ldexpf((float)m, -24);
+        return sign ? -v : v;
+    }
+    if (e == 0x1F) return u2f(sign | 0x7F800000u | (m << 13));
+    return u2f(sign | ((e - 15 + 127) << 23) | (m << 13));
+}
+
+static int bfp16_exponent(float m) {
+    if (!(m > 0.0f)) return 0;
+    int e;
+    frexpf(m, &e);
+    return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+
+static uint16_t bfp16_mantissa(float v, int e) {
+    float q = rintf(ldexpf(v, 15 - e));
+    if (q > 32767.0f) q = 32767.0f;
+    if (q < -32767.0f) q = -32767.0f;
+    return (uint16_t)(int16_t)q;
+}
+
+static float bfp16_value(uint16_t m, int e) { return ldexpf((float)(int16_t)m, e - 15); }
+
+void oracle_encode16(int fmt, const float* src, uint16_t* dst, int8_t* exp, size_t n, float scale) {
+    if (fmt == 3) {
+        for (size_t b = 0; b < n / 16; ++b) {
+            float m = 0.0f;
+            for (int k = 0; k < 16; ++k) m = fmaxf(m, fabsf(src[16 * b + k]));
+            const int e = bfp16_exponent(m);
+            exp[b] = (int8_t)e;
+            for (int k = 0; k < 16; ++k) dst[16 * b + k] = bfp16_mantissa(src[16 * b + k], e);
+        }
+        return;
+    }
+    for (size_t i = 0; i < n; ++i)
+        dst[i] = fmt == 1 ? oracle_f32_to_bf16(src[i]) : oracle_f32_to_fp16(src[i] * scale);
+}
+
+void oracle_decode16(int fmt, const uint16_t* src, const int8_t* exp, float* dst, size_t n,
+                     float scale) {
+    const float inv = 1.0f / scale;
+    for (size_t i = 0; i < n; ++i) {
+        if (fmt == 1) dst[i] = oracle_bf16_to_f32(src[i]);
+        else if (fmt == 2) dst[i] = oracle_fp16_to_f32(src[i]) * inv;
+        else dst[i] = bfp16_value(src[i], exp[i / 16]);
+    }
+}
+
+void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const uint16_t* prev,
+                        const int8_t* prev_e, const uint16_t* vel2, const int8_t* vel2_e,
+                        uint16_t* next, int8_t* next_e, size_t nx, size_t ny, size_t nz, float dx,
+                        float dy, float dz, float dt, float p_scale, float v_scale) {
+    const size_t n = nx * ny * nz;
+    float* fc = (float*)malloc(n * sizeof(float));
+    float* fp = (float*)malloc(n * sizeof(float));
+    float* fv = (float*)malloc(n * sizeof(float));
+    float* fn = (float*)calloc(n, sizeof(float));
+    oracle_decode16(fmt, cur, cur_e, fc, n, p_scale);
+    oracle_decode16(fmt, prev, prev_e, fp, n, p_scale);
+    oracle_decode16(fmt, vel2, vel2_e, fv, n, v_scale);
+    oracle_wave_step_f32(fc, fp, fv, fn, nx, ny, nz, dx, dy, dz, dt);
+    for (size_t z = R; z < nz - R; ++z)
+        for (size_t y = R; y < ny - R; ++y) {
+            const size_t row = (z * ny + y) * nx;
+            if (fmt == 3) {
+                for (size_t b = 0; b < nx / 16; ++b) {
+                    if (16 * b >= nx - R) continue; /* no interior cell in this block */
+                    float m = 0.0f;
+                    for (size_t x = 16 * b; x < 16 * b + 16; ++x)
+                        if (x >= R && x < nx - R) m = fmaxf(m, fabsf(fn[row + x]));
+                    const int e = bfp16_exponent(m);
+                    next_e[(row >> 4) + b] = (int8_t)e;
+                    for (size_t x = 16 * b; x < 16 * b + 16; ++x)
+                        if (x >= R && x < nx - R) next[row + x] = bfp16_mantissa(fn[row + x], e);
+                }
+            } else {
+                for (size_t x = R; x < nx - R; ++x)
+                    next[row + x] = fmt == 1 ? oracle_f32_to_bf16(fn[row + x])
+                                             : oracle_f32_to_fp16(fn[row + x] * p_scale);
+            }
+        }
+    free(fc);
+    free(fp);
+    free(fv);
+    free(fn);
 }

@@ -10,8 +10,9 @@
  * reference's: x fastest, idx = z*nx*ny + y*nx + x, dims include a radius-4
  * zero halo on every face (include/stencil/types.hpp:110-136).
  *
- * How each function is pinned (checked against something other than itself)
- * is listed in oracle/PINNING.md and enforced by tests/test_oracle_*.py.
+ * How each function is pinned (checked against something other than itself:
+ * the reference's own outputs, closed forms, independent implementations) is
+ * enforced by tests/test_oracle.py and tests/test_oracle_prec16.py.
  */
 #ifndef STENCIL_ORACLE_H
 #define STENCIL_ORACLE_H
@@ -99,6 +100,28 @@ double oracle_rel_l2_interior_f32(const float* a, const float* b, size_t nx, siz
                                   size_t nz);
 double oracle_rel_l2_interior_f64f32(const float* a, const double* b, size_t nx, size_t ny,
                                      size_t nz);
+
+/* ---- 16-bit field storage (no reference counterpart: the reference only
+ * has the compile-time FP64 switch).  Formats: 1 bfloat16 (round to nearest
+ * even), 2 IEEE half of value*scale (scale a power of two), 3 block floating
+ * point (int16 mantissa per value, one int8 exponent e per 16 x-consecutive
+ * values, v = m * 2^(e-15), e = frexp exponent of the block's max |v|, 0 for
+ * an all-zero block, clamped to [-127, 127]; |m| <= 32767). ---- */
+uint16_t oracle_f32_to_bf16(float f);
+float oracle_bf16_to_f32(uint16_t h);
+uint16_t oracle_f32_to_fp16(float f);
+float oracle_fp16_to_f32(uint16_t h);
+void oracle_encode16(int fmt, const float* src, uint16_t* dst, int8_t* exp, size_t n, float scale);
+void oracle_decode16(int fmt, const uint16_t* src, const int8_t* exp, float* dst, size_t n,
+                     float scale);
+/* One acoustic step on 16-bit fields: decode p_cur, p_prev, vel2 (exact), the
+ * FP32 step in the reference association (oracle_wave_step_f32), then encode
+ * the interior of p_next; halo cells and the exponents of blocks with no
+ * interior cell are left as they are.  Needs nx % 16 == 0. */
+void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const uint16_t* prev,
+                        const int8_t* prev_e, const uint16_t* vel2, const int8_t* vel2_e,
+                        uint16_t* next, int8_t* next_e, size_t nx, size_t ny, size_t nz, float dx,
+                        float dy, float dz, float dt, float p_scale, float v_scale);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,13 @@ HIP kernels behind the C-ABI in include/stencil_capi.h and the C++ API in
 include/stencil/.  This package is the Python binding of that C-ABI.
 """
 from .api import (  # noqa: F401
+    STORAGE,
+    Field16,
+    Reducer,
+    decode16,
+    encode16,
+    launch_tiled_16,
+    pow2_scale_for,
     VARIANTS,
     Config,
     GridDims,

@@ -47,6 +47,10 @@ class stencil_tile_shape(C.Structure):
                 ("nontemporal", C.c_int)]
 
 
+class stencil_field16(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("exp", C.c_void_p)]
+
+
 class stencil_perf(C.Structure):
     _fields_ = [("time_ms", C.c_double), ("bandwidth_gbs", C.c_double), ("gflops", C.c_double),
                 ("model_bandwidth_gbs", C.c_double), ("grid_size", C.c_uint64),
@@ -79,6 +83,17 @@ SIGNATURES = {
     "stencil_tune_heat_f32": (_I, [_P, _P, _G, _I, C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
     "stencil_tuned_shape": (_I, [_G, _I, C.POINTER(stencil_tile_shape)]),
     "stencil_clear_tuning": (None, []),
+    "stencil_field16_bytes": (_I, [_I, _G, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "stencil_encode16": (_I, [_I, _P, C.POINTER(stencil_field16), C.c_uint64, C.c_float, _P]),
+    "stencil_decode16": (_I, [_I, C.POINTER(stencil_field16), _P, C.c_uint64, C.c_float, _P]),
+    "stencil_launch_tiled_16": (_I, [_I] + [C.POINTER(stencil_field16)] * 4 + [_G, C.c_float, C.c_float,
+                                                                              C.POINTER(stencil_tile_shape), _P]),
+    "stencil_reduce_workspace_bytes": (C.c_uint64, []),
+    "stencil_energy_f32": (_I, [_P, _P, _P, _G, _P, _P, _P]),
+    "stencil_energy_f64": (_I, [_P, _P, _P, _G, _P, _P, _P]),
+    "stencil_diff_norms_f32_f64": (_I, [_P, _P, _G, _P, _P, _P]),
+    "stencil_dot_f64": (_I, [_P, _P, _G, _P, _P, _P]),
+    "stencil_dot_f32": (_I, [_P, _P, _G, _P, _P, _P]),
     "stencil_host_init_gaussian_f32": (_I, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
                                             C.c_float, C.c_float, C.c_float]),
     "stencil_host_layered_noise_vel2_f32": (_I, [_P, C.c_uint64, C.c_uint64, C.c_uint64, _I,

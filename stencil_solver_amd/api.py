@@ -20,7 +20,8 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import StencilError, check, lib, stencil_config, stencil_grid, stencil_perf, stencil_tile_shape
+from ._lib import (StencilError, check, lib, stencil_config, stencil_field16, stencil_grid, stencil_perf,
+                   stencil_tile_shape)
 
 VARIANTS = {"naive": 0, "tiled": 1, "multigpu": 2}
 RADIUS = 4
@@ -223,6 +224,115 @@ def heat_bytes_per_step(grid: GridDims, bytes_per_value: int = 4) -> float:
     return float(lib().stencil_heat_bytes_per_step(C.byref(g), bytes_per_value))
 
 
+# ── 16-bit field storage ────────────────────────────────────────────────────
+STORAGE = {"bf16": 1, "fp16": 2, "bfp16": 3}
+
+
+class Field16:
+    """A field in 16-bit storage on the device (include/stencil_capi.h,
+    ``stencil_field16``): ``data`` int16 tensor of the grid's shape, ``exp``
+    int8 block exponents (block floating point only), ``scale`` the
+    power-of-two field scale (fp16 only)."""
+
+    def __init__(self, fmt: str, shape, device, scale: float = 1.0):
+        import torch
+
+        if fmt not in STORAGE:
+            raise ValueError(f"unknown 16-bit storage {fmt!r}")
+        self.fmt, self.shape, self.scale = fmt, tuple(shape), float(scale)
+        n = int(np.prod(shape))
+        self.data = torch.zeros(shape, dtype=torch.int16, device=device)
+        self.exp = torch.zeros(n // 16, dtype=torch.int8, device=device) if fmt == "bfp16" else None
+
+    def c(self) -> stencil_field16:
+        return stencil_field16(self.data.data_ptr(), self.exp.data_ptr() if self.exp is not None else None)
+
+    def nbytes(self) -> int:
+        return self.data.numel() * 2 + (self.exp.numel() if self.exp is not None else 0)
+
+
+def pow2_scale_for(max_abs: float, target: float = 2.0 ** 14) -> float:
+    """Power-of-two scale that maps max_abs to at most `target` (fp16 storage)."""
+    import math
+
+    if not max_abs > 0:
+        return 1.0
+    return float(2.0 ** math.floor(math.log2(target / max_abs)))
+
+
+def encode16(fmt: str, src, dst: Optional[Field16] = None, scale: float = 1.0, stream=None) -> Field16:
+    import torch
+
+    if dst is None:
+        dst = Field16(fmt, tuple(src.shape), src.device, scale)
+    if src.dtype != torch.float32 or not src.is_contiguous():
+        raise TypeError("encode16 takes a contiguous float32 tensor")
+    f = dst.c()
+    check(lib().stencil_encode16(STORAGE[fmt], src.data_ptr(), C.byref(f), src.numel(), float(dst.scale),
+                                 _stream(stream)), "encode16")
+    return dst
+
+
+def decode16(field: Field16, out=None, stream=None):
+    import torch
+
+    if out is None:
+        out = torch.empty(field.shape, dtype=torch.float32, device=field.data.device)
+    f = field.c()
+    check(lib().stencil_decode16(STORAGE[field.fmt], C.byref(f), out.data_ptr(), out.numel(), float(field.scale),
+                                 _stream(stream)), "decode16")
+    return out
+
+
+def launch_tiled_16(p_cur: Field16, p_prev: Field16, vel2: Field16, p_next: Field16, grid: GridDims,
+                    tile=None, stream=None) -> None:
+    """One acoustic step on 16-bit fields (FP32 arithmetic, reference association)."""
+    fmt = p_cur.fmt
+    if any(f.fmt != fmt for f in (p_prev, vel2, p_next)):
+        raise ValueError("all four fields must share one storage format")
+    if p_prev.scale != p_cur.scale or p_next.scale != p_cur.scale:
+        raise ValueError("p_cur, p_prev and p_next must share one scale")
+    fs = [f.c() for f in (p_cur, p_prev, vel2, p_next)]
+    g = grid.to_c()
+    shp = C.byref(tile) if tile is not None else None
+    check(lib().stencil_launch_tiled_16(STORAGE[fmt], *[C.byref(f) for f in fs], C.byref(g), float(p_cur.scale),
+                                        float(vel2.scale), shp, _stream(stream)), "launch_tiled_16")
+
+
+class Reducer:
+    """Deterministic FP64 reductions over the interior (include/stencil_capi.h)."""
+
+    def __init__(self, device):
+        import torch
+
+        nb = int(lib().stencil_reduce_workspace_bytes())
+        self.work = torch.empty(nb // 8, dtype=torch.float64, device=device)
+        self.out = torch.zeros(3, dtype=torch.float64, device=device)
+
+    def _run(self, fn, ptrs, grid, stream):
+        g = grid.to_c()
+        check(fn(*ptrs, C.byref(g), self.out.data_ptr(), self.work.data_ptr(), _stream(stream)), fn.__name__)
+        return self.out.cpu().tolist()
+
+    def energy(self, p_n, p_np1, vel2, grid: GridDims, stream=None):
+        """(kinetic, potential) of the leapfrog pair; their sum is the conserved energy."""
+        import torch
+
+        fn = lib().stencil_energy_f64 if p_n.dtype == torch.float64 else lib().stencil_energy_f32
+        k, p, _ = self._run(fn, [p_n.data_ptr(), p_np1.data_ptr(), vel2.data_ptr()], grid, stream)
+        return k, p
+
+    def rel_l2(self, a32, b64, grid: GridDims, stream=None) -> float:
+        num, den, _ = self._run(lib().stencil_diff_norms_f32_f64, [a32.data_ptr(), b64.data_ptr()], grid, stream)
+        return (num / den) ** 0.5 if den > 0 else num ** 0.5
+
+    def dot(self, a, b, grid: GridDims, stream=None) -> float:
+        import torch
+
+        fn = lib().stencil_dot_f64 if a.dtype == torch.float64 else lib().stencil_dot_f32
+        return self._run(fn, [a.data_ptr(), b.data_ptr()], grid, stream)[0]
+
+
 # ── Seeded host inputs ──────────────────────────────────────────────────────
 def host_gaussian(shape, cx=0.5, cy=0.5, cz=0.5, sigma=5.0) -> np.ndarray:
     """init_gaussian (reference include/stencil/grid.hpp:57-78) into a new array."""
@@ -366,5 +476,6 @@ def make_solver(cfg: Config) -> Solver:
 
 __all__ = ["GridDims", "Config", "PerfResult", "Solver", "StencilError", "make_solver", "upload_fd_coefficients",
            "launch_naive", "launch_tiled", "launch", "launch_tiled_ex", "tile_shape", "tune_tiled", "tune_heat",
-           "tuned_shape", "clear_tuning", "heat_step", "wave_bytes_per_step",
+           "tuned_shape", "clear_tuning", "Field16", "STORAGE", "encode16", "decode16", "launch_tiled_16",
+           "pow2_scale_for", "Reducer", "heat_step", "wave_bytes_per_step",
            "heat_bytes_per_step", "host_gaussian", "host_layered_noise_vel2", "host_noise_cube", "VARIANTS"]

@@ -203,6 +203,66 @@ int stencil_heat_step_f64(const double* t, double* t_next, double alpha, const s
                                      static_cast<hipStream_t>(stream)));
 }
 
+int stencil_field16_bytes(int storage, const stencil_grid* grid, uint64_t* data_bytes,
+                          uint64_t* exp_bytes) {
+    if (!valid_grid(grid) || storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16)
+        return STENCIL_ERR_INVALID;
+    const uint64_t n = grid->nx * grid->ny * grid->nz;
+    if (data_bytes) *data_bytes = 2 * n;
+    if (exp_bytes) *exp_bytes = storage == STENCIL_STORE_BFP16 ? (n + 15) / 16 : 0;
+    return STENCIL_OK;
+}
+
+namespace {
+bool pow2(float s) {
+    int e = 0;
+    return s > 0.f && std::isfinite(s) && std::frexp(s, &e) == 0.5f;
+}
+bool field16_ok(int storage, const stencil_field16* f) {
+    if (!f || !f->data || (reinterpret_cast<uintptr_t>(f->data) % 16) != 0) return false;
+    return storage != STENCIL_STORE_BFP16 || f->exp != nullptr;
+}
+Field16 as_field(const stencil_field16* f) { return Field16{f->data, f->exp}; }
+}  // namespace
+
+int stencil_encode16(int storage, const float* src, const stencil_field16* dst, uint64_t n,
+                     float scale, void* stream) {
+    if (storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
+    if (!src || !field16_ok(storage, dst) || n == 0 || n % 16 != 0 || !pow2(scale))
+        return STENCIL_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(src) % 16 != 0) return STENCIL_ERR_INVALID;
+    return hip_status(encode16(storage, src, as_field(dst), n, scale, static_cast<hipStream_t>(stream)));
+}
+
+int stencil_decode16(int storage, const stencil_field16* src, float* dst, uint64_t n, float scale,
+                     void* stream) {
+    if (storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
+    if (!dst || !field16_ok(storage, src) || n == 0 || n % 16 != 0 || !pow2(scale))
+        return STENCIL_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(dst) % 16 != 0) return STENCIL_ERR_INVALID;
+    return hip_status(decode16(storage, as_field(src), dst, n, scale, static_cast<hipStream_t>(stream)));
+}
+
+int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
+                            const stencil_field16* vel2, const stencil_field16* p_next,
+                            const stencil_grid* grid, float p_scale, float v_scale,
+                            const stencil_tile_shape* tile, void* stream) {
+    if (storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
+    if (!valid_grid(grid) || !field16_ok(storage, p_cur) || !field16_ok(storage, p_prev) ||
+        !field16_ok(storage, vel2) || !field16_ok(storage, p_next))
+        return STENCIL_ERR_INVALID;
+    if (p_next->data == p_cur->data || p_next->data == p_prev->data || p_next->data == vel2->data)
+        return STENCIL_ERR_INVALID;
+    if (!pow2(p_scale) || !pow2(v_scale)) return STENCIL_ERR_INVALID;
+    const Dims d = dims_of(grid);
+    if (d.nx % 16 != 0 || d.nx * d.ny >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
+    ZqShape shape;
+    if (!shape_of(tile, d, 2 + storage, &shape)) return STENCIL_ERR_INVALID;
+    return hip_status(wave_zq_16(storage, as_field(p_cur), as_field(p_prev), as_field(vel2),
+                                 as_field(p_next), d, scalars_of<float>(grid), p_scale, v_scale, shape,
+                                 static_cast<hipStream_t>(stream)));
+}
+
 double stencil_wave_bytes_per_step(const stencil_grid* g, int bytes_per_value) {
     if (!valid_grid(g) || g->nx <= 8 || g->ny <= 8 || g->nz <= 8) return 0.0;
     return 4.0 * bytes_per_value * double(g->nx - 8) * double(g->ny - 8) * double(g->nz - 8);

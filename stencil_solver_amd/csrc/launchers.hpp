@@ -64,6 +64,22 @@ hipError_t wave_zq_f32(const float* cur, const float* prev, const float* vel2, f
                        const Dims& d, const Scalars<float>& s, bool exact, ZqShape shape,
                        hipStream_t st);
 
+// A field in 16-bit storage: data = nx*ny*nz uint16; exp = nx*ny*nz/16 int8
+// block exponents (block floating point only).
+struct Field16 {
+    void* data;
+    int8_t* exp;
+};
+// The acoustic step on 16-bit fields (fmt: 1 bf16, 2 scaled fp16, 3 block
+// fp16), FP32 arithmetic in the reference association.  p_scale applies to
+// p_cur / p_prev / p_next, v_scale to vel2 (powers of two; fp16 only).
+hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Field16& vel2,
+                      const Field16& next, const Dims& d, const Scalars<float>& s, float p_scale,
+                      float v_scale, ZqShape shape, hipStream_t st);
+// Elementwise conversion between FP32 and a 16-bit format (n % 16 == 0).
+hipError_t encode16(int fmt, const float* src, Field16 dst, size_t n, float scale, hipStream_t st);
+hipError_t decode16(int fmt, Field16 src, float* dst, size_t n, float scale, hipStream_t st);
+
 // The heat step t_next = t + coef * Lap(t), coef = dt2 * alpha formed in float
 // (the reference's dt2 * vel2[c] with vel2 == alpha).
 hipError_t heat_naive_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,

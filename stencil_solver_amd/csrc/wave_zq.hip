@@ -35,10 +35,16 @@ using dev::kR;
 enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3 };
 
 struct ZqArgs {
-    const float* cur;
-    const float* prev;
-    const float* vel2;
-    float* next;
+    const void* cur;
+    const void* prev;
+    const void* vel2;
+    void* next;
+    // 16-bit storage: block exponents (kBfp16) and power-of-two scales (kFp16s).
+    const int8_t* cur_e;
+    const int8_t* prev_e;
+    const int8_t* vel2_e;
+    int8_t* next_e;
+    float p_scale, p_inv, v_inv;
     int nx, ny, nz;
     int tiles_x, tiles_y, chunk;
     int xcd_remap;
@@ -47,20 +53,58 @@ struct ZqArgs {
     float w0, wx[4], wy[4], wz[4];   // fused weights (heat: pre-multiplied by coef)
 };
 
-__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+using dev::Fmt;
 
-template <bool NT>
-__device__ __forceinline__ float4 ld_stream(const float* p) {
-    if constexpr (NT) return dev::load_stream(reinterpret_cast<const float4*>(p));
-    return ld4(p);
-}
-template <bool NT>
-__device__ __forceinline__ void st_stream(float* p, float4 v) {
-    if constexpr (NT) dev::store_stream(reinterpret_cast<float4*>(p), v);
-    else *reinterpret_cast<float4*>(p) = v;
+// Loads the 4 values at element index idx (a multiple of 4) of a field in
+// storage format FMT and returns them as float (decoding is exact).
+template <int FMT, bool NT>
+__device__ __forceinline__ float4 load4(const void* base, const int8_t* ebase, size_t idx, float inv) {
+    if constexpr (FMT == dev::kF32) {
+        const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(base) + idx);
+        if constexpr (NT) return dev::load_stream(p);
+        return *p;
+    } else {
+        const uint2* p = reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(base) + idx);
+        const uint2 u = NT ? dev::load_stream_u2(p) : *p;
+        if constexpr (FMT == dev::kBf16) return dev::bf16x4_to_float4(u);
+        if constexpr (FMT == dev::kFp16s) return dev::fp16x4_to_float4(u, inv);
+        return dev::bfp16x4_to_float4(u, ebase[idx >> 4]);
+    }
 }
 
-template <int TXV, int TY, int MODE, bool NT>
+// Stores 4 values at element index idx.  Every lane of a wave reaches this
+// (kBfp16 shares one exponent across a quad of lanes); `vals` says whether this
+// lane writes its values, `blk` whether its quad's block holds outputs.
+template <int FMT, bool NT>
+__device__ __forceinline__ void store4(void* base, int8_t* ebase, size_t idx, float4 v, float scale,
+                                       bool vals, bool blk) {
+    if constexpr (FMT == dev::kF32) {
+        if (vals) {
+            float4* p = reinterpret_cast<float4*>(static_cast<float*>(base) + idx);
+            if constexpr (NT) dev::store_stream(p, v);
+            else *p = v;
+        }
+    } else if constexpr (FMT == dev::kBfp16) {
+        const float m = vals ? dev::max_abs4(v) : 0.f;
+        const int e = dev::bfp16_exponent(dev::quad_max(m));
+        uint2* p = reinterpret_cast<uint2*>(static_cast<uint16_t*>(base) + idx);
+        const uint2 u = dev::float4_to_bfp16x4(v, e);
+        if (vals) {
+            if constexpr (NT) dev::store_stream_u2(p, u);
+            else *p = u;
+        }
+        if (blk && (threadIdx.x & 3) == 0) ebase[idx >> 4] = static_cast<int8_t>(e);
+    } else {
+        if (vals) {
+            const uint2 u = FMT == dev::kBf16 ? dev::float4_to_bf16x4(v) : dev::float4_to_fp16x4(v, scale);
+            uint2* p = reinterpret_cast<uint2*>(static_cast<uint16_t*>(base) + idx);
+            if constexpr (NT) dev::store_stream_u2(p, u);
+            else *p = u;
+        }
+    }
+}
+
+template <int TXV, int TY, int MODE, bool NT, int FMT = dev::kF32>
 __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
     constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact);
@@ -127,17 +171,23 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
 
     // Predicated loads; the zero is built in registers so the compiler cannot
     // turn the predicate into a select between a global and a private address.
-    const float* const cur_base = a.cur;
-    auto cur_at = [cur_base, plane](int z, unsigned off, bool ok) -> float4 {
+    const void* const cur_base = a.cur;
+    const int8_t* const cur_e = a.cur_e;
+    const float p_inv = a.p_inv;
+    auto cur_at = [cur_base, cur_e, p_inv, plane](int z, unsigned off, bool ok) -> float4 {
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok) v = ld4(cur_base + static_cast<size_t>(z) * plane + off);
+        if (ok) v = load4<FMT, false>(cur_base, cur_e, static_cast<size_t>(z) * plane + off, p_inv);
         return v;
     };
-    auto stream_at = [plane, own_off, own_out](const float* f, int z) -> float4 {
+    auto stream_at = [plane, own_off, own_out](const void* f, const int8_t* fe, float inv, int z) -> float4 {
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own_out) v = ld_stream<NT>(f + static_cast<size_t>(z) * plane + own_off);
+        if (own_out) v = load4<FMT, NT>(f, fe, static_cast<size_t>(z) * plane + own_off, inv);
         return v;
     };
+    // This lane's quad (16 x-consecutive cells) holds output cells: it owns
+    // the block exponent of kBfp16 storage.
+    const int xq = x & ~15;
+    const bool blk_out = (y >= kR) && (y < a.ny - kR) && (xq < a.nx - kR);
 
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
     float4 q[9];
@@ -148,8 +198,8 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
     for (int k = 0; k < kHaloPer; ++k) h_cur[k] = cur_at(z_begin, h_off[k], h_in[k]);
     float4 pv_cur = make_float4(0.f, 0.f, 0.f, 0.f), vv_cur = pv_cur;
     if constexpr (!kHeat) {
-        pv_cur = stream_at(a.prev, z_begin);
-        vv_cur = stream_at(a.vel2, z_begin);
+        pv_cur = stream_at(a.prev, a.prev_e, a.p_inv, z_begin);
+        vv_cur = stream_at(a.vel2, a.vel2_e, a.v_inv, z_begin);
     }
 
     for (int zb = z_begin; zb < z_end; zb += 9) {
@@ -165,8 +215,8 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
                 for (int k = 0; k < kHaloPer; ++k) h_nxt[k] = cur_at(zz + 1, h_off[k], h_in[k]);
                 float4 pv_nxt = make_float4(0.f, 0.f, 0.f, 0.f), vv_nxt = pv_nxt;
                 if constexpr (!kHeat) {
-                    pv_nxt = stream_at(a.prev, zz + 1);
-                    vv_nxt = stream_at(a.vel2, zz + 1);
+                    pv_nxt = stream_at(a.prev, a.prev_e, a.p_inv, zz + 1);
+                    vv_nxt = stream_at(a.vel2, a.vel2_e, a.v_inv, zz + 1);
                 }
 
                 const int b = (zz - z_begin) & 1;
@@ -177,6 +227,7 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
                     if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h_cur[k];
                 __syncthreads();
 
+                float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (own_out) {
                     const float4 xl = lds[b][ty + kR][tx];
                     const float4 xr = lds[b][ty + kR][tx + 2];
@@ -188,7 +239,6 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
                         ym[r - 1] = lds[b][ty + kR - r][tx + 1];
                         yp[r - 1] = lds[b][ty + kR + r][tx + 1];
                     }
-                    float4 out;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float p = w[4 + i];
@@ -234,8 +284,9 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
                         }
                         dev::set_comp(out, i, res);
                     }
-                    st_stream<NT>(a.next + static_cast<size_t>(zz) * plane + own_off, out);
                 }
+                store4<FMT, NT>(a.next, a.next_e, static_cast<size_t>(zz) * plane + own_off, out,
+                                a.p_scale, own_out, blk_out);
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k) h_cur[k] = h_nxt[k];
                 pv_cur = pv_nxt;
@@ -273,6 +324,26 @@ KernelFn pick(int txv, int ty, int mode, bool nt) {
     }
 }
 
+template <int FMT, bool NT>
+KernelFn pick_shape16(int txv, int ty) {
+    if (txv == 16 && ty == 16) return zq_kernel<16, 16, kWaveExact, NT, FMT>;
+    if (txv == 16 && ty == 32) return zq_kernel<16, 32, kWaveExact, NT, FMT>;
+    if (txv == 32 && ty == 8) return zq_kernel<32, 8, kWaveExact, NT, FMT>;
+    if (txv == 32 && ty == 16) return zq_kernel<32, 16, kWaveExact, NT, FMT>;
+    if (txv == 64 && ty == 4) return zq_kernel<64, 4, kWaveExact, NT, FMT>;
+    if (txv == 64 && ty == 8) return zq_kernel<64, 8, kWaveExact, NT, FMT>;
+    return nullptr;
+}
+
+KernelFn pick16(int txv, int ty, int fmt, bool nt) {
+    switch (fmt) {
+        case dev::kBf16: return nt ? pick_shape16<dev::kBf16, true>(txv, ty) : pick_shape16<dev::kBf16, false>(txv, ty);
+        case dev::kFp16s: return nt ? pick_shape16<dev::kFp16s, true>(txv, ty) : pick_shape16<dev::kFp16s, false>(txv, ty);
+        case dev::kBfp16: return nt ? pick_shape16<dev::kBfp16, true>(txv, ty) : pick_shape16<dev::kBfp16, false>(txv, ty);
+        default: return nullptr;
+    }
+}
+
 // Fused weights: c_r / h^2 formed in double from the exact rationals and the
 // launcher's float 1/h^2, rounded once.
 void fill_weights(ZqArgs& a, double scale) {
@@ -286,10 +357,12 @@ void fill_weights(ZqArgs& a, double scale) {
     }
 }
 
-hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st) {
+hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st,
+                     int fmt = dev::kF32) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
     shape = zq_resolve(d, shape);
-    KernelFn fn = pick(shape.txv, shape.ty, mode, shape.nt != 0);
+    KernelFn fn = fmt == dev::kF32 ? pick(shape.txv, shape.ty, mode, shape.nt != 0)
+                                   : pick16(shape.txv, shape.ty, fmt, shape.nt != 0);
     if (!fn) return hipErrorInvalidValue;
     const int64_t inz = d.nz - 2 * kR;
     a.nx = static_cast<int>(d.nx);
@@ -367,6 +440,25 @@ hipError_t wave_zq_f32(const float* cur, const float* prev, const float* vel2, f
     a.s = s;
     fill_weights(a, 1.0);
     return launch_zq(a, d, shape, exact ? kWaveExact : kWaveFused, st);
+}
+
+hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Field16& vel2,
+                      const Field16& next, const Dims& d, const Scalars<float>& s, float p_scale,
+                      float v_scale, ZqShape shape, hipStream_t st) {
+    ZqArgs a{};
+    a.cur = cur.data;
+    a.prev = prev.data;
+    a.vel2 = vel2.data;
+    a.next = next.data;
+    a.cur_e = cur.exp;
+    a.prev_e = prev.exp;
+    a.vel2_e = vel2.exp;
+    a.next_e = next.exp;
+    a.p_scale = p_scale;
+    a.p_inv = 1.0f / p_scale;
+    a.v_inv = 1.0f / v_scale;
+    a.s = s;
+    return launch_zq(a, d, shape, kWaveExact, st, fmt);
 }
 
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,
