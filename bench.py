@@ -282,7 +282,8 @@ def precision_study(ss, p0, vel2, n, steps):
         i = 0
         step_fn(b[(i + 1) % 3], b[i], b[(i + 2) % 3])
         i = 1
-        e_first = energy(decode(b[0]), decode(b[1]), v_for_energy) if v_for_energy is not None else None
+        # levels after the first step: p^0 = b[1], p^1 = b[2]
+        e_first = energy(decode(b[1]), decode(b[2]), v_for_energy) if v_for_energy is not None else None
         torch.cuda.synchronize()
         t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0.record()
