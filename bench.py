@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512, help="cells per axis (halo included)")
     ap.add_argument("--variant", default="tiled", choices=["tiled", "naive", "tiled_fused"])
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16", "bfp16"],
+                    help="field storage of the timed loop: FP32, or 16-bit storage with FP32 "
+                         "arithmetic (tiled kernel, fused-weight form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the side measurements")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
@@ -84,6 +87,32 @@ def main():
             return lambda c, p, v, o: ss.launch_tiled_ex(c, p, v, o, grid, exact=False)
         return lambda c, p, v, o: ss.launch_tiled(c, p, v, o, grid)
 
+    def timed_run16(fmt, steps, warmup):
+        vs = ss.pow2_scale_for(float(vel2.max())) if fmt == "fp16" else 1.0
+        gv = ss.encode16(fmt, vel2, ss.Field16(fmt, shape, dev, vs))
+        bufs = [ss.encode16(fmt, x, ss.Field16(fmt, shape, dev, 1.0)) for x in (p0, p0, torch.zeros_like(p0))]
+        if not args.no_tune:
+            tb = ss.Field16(fmt, shape, dev, 1.0)
+            print(f"[bench] tuned {fmt} shape {ss.tune_tiled_16(bufs[1], bufs[0], gv, tb, grid, reps=5)}",
+                  file=sys.stderr, flush=True)
+            del tb
+        i = 0
+        for _ in range(warmup):
+            ss.launch_tiled_16(bufs[(i + 1) % 3], bufs[i], gv, bufs[(i + 2) % 3], grid)
+            i = (i + 1) % 3
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            ss.launch_tiled_16(bufs[(i + 1) % 3], bufs[i], gv, bufs[(i + 2) % 3], grid)
+            i = (i + 1) % 3
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        out = ss.decode16(bufs[(i + 1) % 3])
+        del bufs, gv
+        return ms, out
+
     def timed_run(variant, steps, warmup):
         fn = launcher(variant)
         bufs = [p0.clone(), p0.clone(), torch.zeros_like(p0)]
@@ -112,13 +141,21 @@ def main():
         tuned = ss.tune_tiled(tb[1], tb[0], vel2, tb[2], grid, reps=5, csv_path=args.tune_csv)
         del tb
         print(f"[bench] tuned shape {tuned}", file=sys.stderr, flush=True)
-    shape_used, _ = ss.tuned_shape(grid)
+    shape_used, _ = ss.tuned_shape(grid, "wave" if args.storage == "fp32" else args.storage)
 
-    ms, out = timed_run(args.variant, args.steps, args.warmup)
+    if args.storage == "fp32":
+        ms, out = timed_run(args.variant, args.steps, args.warmup)
+        bytes_pt = 16.0
+    else:
+        if args.variant != "tiled":
+            sys.exit("--storage applies to the tiled kernel only")
+        ms, out = timed_run16(args.storage, args.steps, args.warmup)
+        bytes_pt = 8.25 if args.storage == "bfp16" else 8.0  # + 1 exponent byte / 16 values
+        args.no_extras = True
     gpts = interior / (ms * 1e-3) / 1e9
-    alg_bytes = 16.0 * interior
+    alg_bytes = bytes_pt * interior
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    print(f"[bench] {args.variant}: {ms:.4f} ms/step, {gpts:.1f} Gpts/s, {achieved:.0f} GB/s",
+    print(f"[bench] {args.variant} ({args.storage}): {ms:.4f} ms/step, {gpts:.1f} Gpts/s, {achieved:.0f} GB/s",
           file=sys.stderr, flush=True)
 
     result = {
@@ -138,7 +175,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"acoustic {n}^3 FP32 leapfrog, 8th-order Laplacian, 16-layer +-5% velocity, "
                                f"Gaussian sigma=3, h=10 m, dt=1e-3 s",
-                   "grid": n, "interior_points": interior, "kernel": args.variant,
+                   "grid": n, "interior_points": interior, "kernel": args.variant, "storage": args.storage,
                    "tile_shape": shape_used if args.variant == "tiled" else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -147,7 +184,8 @@ def main():
 
     # Measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
     # scaled by a known-size copy), committed under profiles/ for this kernel.
-    tfile = ROOT / "profiles" / f"traffic_wave_{args.variant}_{n}.json"
+    tag = args.variant if args.storage == "fp32" else f"{args.variant}_{args.storage}"
+    tfile = ROOT / "profiles" / f"traffic_wave_{tag}_{n}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
         result["roofline"]["traffic"] = t.get("bytes_per_launch")
@@ -226,7 +264,7 @@ def main():
                 pshape = (pn, pn, pn)
                 pp0 = torch.from_numpy(ss.host_gaussian(pshape, 0.5, 0.5, 0.5, 3.0)).to(dev)
                 pv2 = torch.from_numpy(ss.host_layered_noise_vel2(pshape, 16, 1500.0, 4000.0, 0.05, 2)).to(dev)
-            result["precision"][str(pn)] = precision_study(ss, pp0, pv2, pn, args.steps)
+            result["precision"][str(pn)] = precision_study(ss, pp0, pv2, pn, args.steps, tune=not args.no_tune)
             del pp0, pv2
         torch.cuda.empty_cache()
         p512 = result["precision"].get(str(n))
@@ -257,7 +295,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def precision_study(ss, p0, vel2, n, steps):
+def precision_study(ss, p0, vel2, n, steps, tune=True):
     """16-bit field storage against an FP64 run of the same seeded inputs:
     `steps` steps of the acoustic workload (h = 10 m, dt = 1e-3 s); for FP32 and
     each 16-bit format: Gpts/s, relative L2 and max-abs error of the final p
@@ -321,12 +359,34 @@ def precision_study(ss, p0, vel2, n, steps):
         vs = ss.pow2_scale_for(float(vel2.max())) if fmt == "fp16" else 1.0
         gv = ss.encode16(fmt, vel2, ss.Field16(fmt, shape, dev, vs))
         vdec = ss.decode16(gv)
+        if tune:
+            tb = [ss.encode16(fmt, p0, ss.Field16(fmt, shape, dev, 1.0)) for _ in range(3)]
+            ss.tune_tiled_16(tb[1], tb[0], gv, tb[2], grid, reps=3)
+            del tb
         last, ms, ea, eb = ring(lambda c, p, o: ss.launch_tiled_16(c, p, gv, o, grid),
                                 lambda: ss.encode16(fmt, p0, ss.Field16(fmt, shape, dev, 1.0)),
                                 lambda: ss.encode16(fmt, torch.zeros_like(p0), ss.Field16(fmt, shape, dev, 1.0)),
                                 decode=ss.decode16, v_for_energy=vdec)
         report(fmt, ss.decode16(last), ms, ea, eb, 8.25 if fmt == "bfp16" else 8.0)
-        del gv, vdec, last
+        out[fmt]["form"] = "fused"
+        out[fmt]["tile_shape"] = ss.tuned_shape(grid, fmt)[0]
+        out[fmt]["speedup_vs_fp32"] = round(out["fp32"]["ms"] / ms, 3)
+        del last
+        # the bit-reproducible reference-association form, timed on a short run
+        b = [ss.encode16(fmt, p0, ss.Field16(fmt, shape, dev, 1.0)) for _ in range(3)]
+        for _ in range(3):
+            ss.launch_tiled_16(b[1], b[0], gv, b[2], grid, exact=True)
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        i = 0
+        for _ in range(50):
+            ss.launch_tiled_16(b[(i + 1) % 3], b[i], gv, b[(i + 2) % 3], grid, exact=True)
+            i = (i + 1) % 3
+        t1.record()
+        torch.cuda.synchronize()
+        out[fmt]["gpts_exact_form"] = round(interior / (t0.elapsed_time(t1) / 50 * 1e-3) / 1e9, 2)
+        del gv, vdec, b
     del ref, v64
     print(f"[bench] precision {n}^3 x {steps}: " +
           ", ".join(f"{k}: {v.get('gpts', '-')} Gpts/s rel {v.get('rel_l2_vs_fp64', 0):.2e}" for k, v in out.items()

@@ -144,7 +144,8 @@ int stencil_tune_tiled_f32(const float* p_cur, const float* p_prev, const float*
 int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* grid, int reps,
                           stencil_tile_shape* best, const char* csv_path, void* hip_stream);
 /* The shape a NULL-shape launch would use for these dims (kind 0 wave, 1
- * heat): returns 1 if it came from the tuner cache, 0 for the default. */
+ * heat, 2 + storage for the 16-bit wave step): returns 1 if it came from the
+ * tuner cache, 0 for the default. */
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
 void stencil_clear_tuning(void);
 
@@ -155,14 +156,17 @@ void stencil_clear_tuning(void);
  * array, n % 4 == 0, 16-byte aligned. */
 int stencil_stream_f32(int op, float* dst, const float* src, uint64_t n, void* hip_stream);
 
-/* ---- 16-bit field storage (config "precision"): p, p_prev and vel2 stored in
- * 16 bits, arithmetic in FP32 with the reference association.  Decoding is
- * exact, so each format is bit-reproducible against a CPU restatement.
+/* ---- 16-bit field storage: p, p_prev and vel2 stored in 16 bits, arithmetic
+ * in FP32.  Decoding is exact; with exact = 1 the step uses the reference
+ * association and is bit-reproducible against a CPU restatement, with exact = 0
+ * it uses the fused-weight form (fewer operations per point; the FP32
+ * arithmetic differs by rounding only, far below the 16-bit storage error).
  *   STENCIL_STORE_BF16  bfloat16, round to nearest even
  *   STENCIL_STORE_FP16  IEEE half of value*scale, scale a power of two per field
  *   STENCIL_STORE_BFP16 block floating point: int16 mantissas + one int8
  *                       exponent per 16 x-consecutive values (v = m*2^(e-15),
- *                       e = frexp exponent of the block max); needs the
+ *                       e = frexp exponent of the block max, clamped to
+ *                       [-112, 127]); needs the
  *                       zero halo of Dirichlet walls, and nx % 16 == 0. ---- */
 enum stencil_storage {
     STENCIL_STORE_BF16 = 1,
@@ -187,8 +191,15 @@ int stencil_decode16(int storage, const stencil_field16* src, float* dst, uint64
  * scale of p_cur / p_prev / p_next, v_scale of vel2 (both 1 unless FP16). */
 int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
                             const stencil_field16* vel2, const stencil_field16* p_next,
-                            const stencil_grid* grid, float p_scale, float v_scale,
+                            const stencil_grid* grid, float p_scale, float v_scale, int exact,
                             const stencil_tile_shape* tile, void* hip_stream);
+/* Autotuner of the 16-bit step (fused form), as stencil_tune_tiled_f32: the
+ * best shape is cached for NULL-shape stencil_launch_tiled_16 calls on this
+ * grid and storage (the sweep CSV has naive_ms = 0). */
+int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
+                          const stencil_field16* vel2, const stencil_field16* p_next,
+                          const stencil_grid* grid, float p_scale, float v_scale, int reps,
+                          stencil_tile_shape* best, const char* csv_path, void* hip_stream);
 
 /* ---- FP64 reductions over the interior (deterministic: fixed-order two-pass
  * reduction through `work`, stencil_reduce_workspace_bytes() of device

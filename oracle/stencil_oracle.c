@@ -377,7 +377,7 @@ static int bfp16_exponent(float m) {
     if (!(m > 0.0f)) return 0;
     int e;
     frexpf(m, &e);
-    return e < -127 ? -127 : (e > 127 ? 127 : e);
+    return e < -112 ? -112 : (e > 127 ? 127 : e);
 }
 
 static uint16_t bfp16_mantissa(float v, int e) {

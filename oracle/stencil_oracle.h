@@ -106,7 +106,8 @@ double oracle_rel_l2_interior_f64f32(const float* a, const double* b, size_t nx,
  * even), 2 IEEE half of value*scale (scale a power of two), 3 block floating
  * point (int16 mantissa per value, one int8 exponent e per 16 x-consecutive
  * values, v = m * 2^(e-15), e = frexp exponent of the block's max |v|, 0 for
- * an all-zero block, clamped to [-127, 127]; |m| <= 32767). ---- */
+ * an all-zero block, clamped to [-112, 127] so that 2^(15-e) and 2^(e-15)*32767
+ * are normal floats; |m| <= 32767). ---- */
 uint16_t oracle_f32_to_bf16(float f);
 float oracle_bf16_to_f32(uint16_t h);
 uint16_t oracle_f32_to_fp16(float f);

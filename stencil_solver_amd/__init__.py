@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     decode16,
     encode16,
     launch_tiled_16,
+    tune_tiled_16,
     pow2_scale_for,
     VARIANTS,
     Config,

@@ -86,8 +86,10 @@ SIGNATURES = {
     "stencil_field16_bytes": (_I, [_I, _G, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "stencil_encode16": (_I, [_I, _P, C.POINTER(stencil_field16), C.c_uint64, C.c_float, _P]),
     "stencil_decode16": (_I, [_I, C.POINTER(stencil_field16), _P, C.c_uint64, C.c_float, _P]),
-    "stencil_launch_tiled_16": (_I, [_I] + [C.POINTER(stencil_field16)] * 4 + [_G, C.c_float, C.c_float,
+    "stencil_launch_tiled_16": (_I, [_I] + [C.POINTER(stencil_field16)] * 4 + [_G, C.c_float, C.c_float, _I,
                                                                               C.POINTER(stencil_tile_shape), _P]),
+    "stencil_tune_tiled_16": (_I, [_I] + [C.POINTER(stencil_field16)] * 4 + [_G, C.c_float, C.c_float, _I,
+                                                                            C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
     "stencil_reduce_workspace_bytes": (C.c_uint64, []),
     "stencil_energy_f32": (_I, [_P, _P, _P, _G, _P, _P, _P]),
     "stencil_energy_f64": (_I, [_P, _P, _P, _G, _P, _P, _P]),

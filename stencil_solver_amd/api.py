@@ -201,10 +201,12 @@ def tune_heat(t, t_next, grid: GridDims, reps: int = 5, csv_path=None, stream=No
 
 
 def tuned_shape(grid: GridDims, kind: str = "wave"):
-    """(shape dict, from_tuner) a NULL-shape launch would use on these dims."""
+    """(shape dict, from_tuner) a NULL-shape launch would use on these dims;
+    kind "wave", "heat", or a 16-bit storage name for launch_tiled_16."""
     shp = stencil_tile_shape()
     g = grid.to_c()
-    r = lib().stencil_tuned_shape(C.byref(g), {"wave": 0, "heat": 1}[kind], C.byref(shp))
+    kinds = {"wave": 0, "heat": 1, "bf16": 3, "fp16": 4, "bfp16": 5}
+    r = lib().stencil_tuned_shape(C.byref(g), kinds[kind], C.byref(shp))
     if r < 0 or r > 1:
         check(r, "tuned_shape")
     return _shape_dict(shp), bool(r)
@@ -285,8 +287,10 @@ def decode16(field: Field16, out=None, stream=None):
 
 
 def launch_tiled_16(p_cur: Field16, p_prev: Field16, vel2: Field16, p_next: Field16, grid: GridDims,
-                    tile=None, stream=None) -> None:
-    """One acoustic step on 16-bit fields (FP32 arithmetic, reference association)."""
+                    tile=None, stream=None, exact: bool = False) -> None:
+    """One acoustic step on 16-bit fields, FP32 arithmetic.  exact=True uses the
+    reference association (bit-reproducible against the CPU restatement);
+    the default fused-weight form differs by FP32 rounding only."""
     fmt = p_cur.fmt
     if any(f.fmt != fmt for f in (p_prev, vel2, p_next)):
         raise ValueError("all four fields must share one storage format")
@@ -296,7 +300,25 @@ def launch_tiled_16(p_cur: Field16, p_prev: Field16, vel2: Field16, p_next: Fiel
     g = grid.to_c()
     shp = C.byref(tile) if tile is not None else None
     check(lib().stencil_launch_tiled_16(STORAGE[fmt], *[C.byref(f) for f in fs], C.byref(g), float(p_cur.scale),
-                                        float(vel2.scale), shp, _stream(stream)), "launch_tiled_16")
+                                        float(vel2.scale), int(bool(exact)), shp, _stream(stream)),
+          "launch_tiled_16")
+
+
+def tune_tiled_16(p_cur: Field16, p_prev: Field16, vel2: Field16, p_next: Field16, grid: GridDims,
+                  reps: int = 5, csv_path=None, stream=None) -> dict:
+    """Autotune launch_tiled_16 (fused form) for this grid and storage format
+    (p_next is overwritten); later NULL-shape calls use the result."""
+    fmt = p_cur.fmt
+    if any(f.fmt != fmt for f in (p_prev, vel2, p_next)):
+        raise ValueError("all four fields must share one storage format")
+    shp = stencil_tile_shape()
+    fs = [f.c() for f in (p_cur, p_prev, vel2, p_next)]
+    g = grid.to_c()
+    path = str(csv_path).encode() if csv_path else None
+    check(lib().stencil_tune_tiled_16(STORAGE[fmt], *[C.byref(f) for f in fs], C.byref(g), float(p_cur.scale),
+                                      float(vel2.scale), int(reps), C.byref(shp), path, _stream(stream)),
+          "tune_tiled_16")
+    return _shape_dict(shp)
 
 
 class Reducer:
@@ -477,5 +499,6 @@ def make_solver(cfg: Config) -> Solver:
 __all__ = ["GridDims", "Config", "PerfResult", "Solver", "StencilError", "make_solver", "upload_fd_coefficients",
            "launch_naive", "launch_tiled", "launch", "launch_tiled_ex", "tile_shape", "tune_tiled", "tune_heat",
            "tuned_shape", "clear_tuning", "Field16", "STORAGE", "encode16", "decode16", "launch_tiled_16",
+           "tune_tiled_16",
            "pow2_scale_for", "Reducer", "heat_step", "wave_bytes_per_step",
            "heat_bytes_per_step", "host_gaussian", "host_layered_noise_vel2", "host_noise_cube", "VARIANTS"]

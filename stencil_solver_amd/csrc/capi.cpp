@@ -245,7 +245,7 @@ int stencil_decode16(int storage, const stencil_field16* src, float* dst, uint64
 
 int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
                             const stencil_field16* vel2, const stencil_field16* p_next,
-                            const stencil_grid* grid, float p_scale, float v_scale,
+                            const stencil_grid* grid, float p_scale, float v_scale, int exact,
                             const stencil_tile_shape* tile, void* stream) {
     if (storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
     if (!valid_grid(grid) || !field16_ok(storage, p_cur) || !field16_ok(storage, p_prev) ||
@@ -255,11 +255,11 @@ int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const ste
         return STENCIL_ERR_INVALID;
     if (!pow2(p_scale) || !pow2(v_scale)) return STENCIL_ERR_INVALID;
     const Dims d = dims_of(grid);
-    if (d.nx % 16 != 0 || d.nx * d.ny >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
+    if (d.nx % 16 != 0 || d.nx * d.ny * 2 >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
     ZqShape shape;
     if (!shape_of(tile, d, 2 + storage, &shape)) return STENCIL_ERR_INVALID;
     return hip_status(wave_zq_16(storage, as_field(p_cur), as_field(p_prev), as_field(vel2),
-                                 as_field(p_next), d, scalars_of<float>(grid), p_scale, v_scale, shape,
+                                 as_field(p_next), d, scalars_of<float>(grid), p_scale, v_scale, exact != 0, shape,
                                  static_cast<hipStream_t>(stream)));
 }
 

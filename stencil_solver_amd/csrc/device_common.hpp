@@ -57,66 +57,88 @@ __device__ __forceinline__ void store_stream_u2(uint2* p, uint2 v) {
 // kFp16s: IEEE half of value * scale, scale a power of two per field.
 // kBfp16: block floating point: int16 mantissas, one shared power-of-two
 //         exponent e (int8) per 16 x-consecutive values: v = m * 2^(e-15),
-//         e = frexp exponent of the block's max |v|, |m| <= 32767.
+//         e = frexp exponent of the block's max |v| clamped to [-112, 127]
+//         (0 for an all-zero block), |m| <= 32767.
+// Stores use gfx950's packed converts (v_cvt_pk_bf16_f32, v_cvt_pk_f16_f32:
+// two values per instruction, round to nearest even); loads are one shift,
+// mask or SDWA convert per value.
 enum Fmt : int { kF32 = 0, kBf16 = 1, kFp16s = 2, kBfp16 = 3 };
 
-__device__ __forceinline__ float bf16_to_float(unsigned h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ unsigned float_to_bf16(float f) {
-    const unsigned b = __float_as_uint(f);
-    return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+    const f32x2 v = {a, b};
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    unsigned r;
+    __builtin_memcpy(&r, &h, 4);
+    return r;
 }
+__device__ __forceinline__ unsigned pack_fp16(float a, float b) {
+    const f32x2 v = {a, b};
+    const f16x2 h = __builtin_convertvector(v, f16x2);
+    unsigned r;
+    __builtin_memcpy(&r, &h, 4);
+    return r;
+}
+__device__ __forceinline__ float half_lo(unsigned u) {
+    const unsigned short s = static_cast<unsigned short>(u);
+    _Float16 f;
+    __builtin_memcpy(&f, &s, 2);
+    return static_cast<float>(f);
+}
+__device__ __forceinline__ float half_hi(unsigned u) { return half_lo(u >> 16); }
+
 __device__ __forceinline__ float4 bf16x4_to_float4(uint2 u) {
     return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u),
                        __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xFFFF0000u));
 }
 __device__ __forceinline__ uint2 float4_to_bf16x4(float4 v) {
-    return make_uint2(float_to_bf16(v.x) | (float_to_bf16(v.y) << 16),
-                      float_to_bf16(v.z) | (float_to_bf16(v.w) << 16));
+    return make_uint2(pack_bf16(v.x, v.y), pack_bf16(v.z, v.w));
 }
 
-__device__ __forceinline__ float half_bits_to_float(unsigned h) {
-    const unsigned short s = static_cast<unsigned short>(h);
-    _Float16 f;
-    __builtin_memcpy(&f, &s, 2);
-    return static_cast<float>(f);
-}
-__device__ __forceinline__ unsigned float_to_half_bits(float f) {
-    const _Float16 h = static_cast<_Float16>(f);
-    unsigned short s;
-    __builtin_memcpy(&s, &h, 2);
-    return s;
+// Half values as stored (still multiplied by the field's scale).
+__device__ __forceinline__ float4 fp16x4_raw(uint2 u) {
+    return make_float4(half_lo(u.x), half_hi(u.x), half_lo(u.y), half_hi(u.y));
 }
 __device__ __forceinline__ float4 fp16x4_to_float4(uint2 u, float inv) {
-    return make_float4(half_bits_to_float(u.x & 0xFFFFu) * inv, half_bits_to_float(u.x >> 16) * inv,
-                       half_bits_to_float(u.y & 0xFFFFu) * inv, half_bits_to_float(u.y >> 16) * inv);
+    const float4 r = fp16x4_raw(u);
+    return make_float4(r.x * inv, r.y * inv, r.z * inv, r.w * inv);
+}
+__device__ __forceinline__ uint2 fp16x4_pack(float4 v) {
+    return make_uint2(pack_fp16(v.x, v.y), pack_fp16(v.z, v.w));
 }
 __device__ __forceinline__ uint2 float4_to_fp16x4(float4 v, float scale) {
-    return make_uint2(float_to_half_bits(v.x * scale) | (float_to_half_bits(v.y * scale) << 16),
-                      float_to_half_bits(v.z * scale) | (float_to_half_bits(v.w * scale) << 16));
+    return make_uint2(pack_fp16(v.x * scale, v.y * scale), pack_fp16(v.z * scale, v.w * scale));
 }
 
-__device__ __forceinline__ float bfp16_value(unsigned m16, int e) {
-    return __builtin_ldexpf(static_cast<float>(static_cast<short>(m16)), e - 15);
-}
 __device__ __forceinline__ float4 bfp16x4_to_float4(uint2 u, int e) {
-    return make_float4(bfp16_value(u.x & 0xFFFFu, e), bfp16_value(u.x >> 16, e),
-                       bfp16_value(u.y & 0xFFFFu, e), bfp16_value(u.y >> 16, e));
+    const int k = e - 15;
+    return make_float4(__builtin_ldexpf(static_cast<float>(static_cast<short>(u.x)), k),
+                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.x >> 16)), k),
+                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.y)), k),
+                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.y >> 16)), k));
 }
-__device__ __forceinline__ unsigned bfp16_mantissa(float v, int e) {
-    float q = __builtin_rintf(__builtin_ldexpf(v, 15 - e));
-    q = q > 32767.f ? 32767.f : (q < -32767.f ? -32767.f : q);
-    return static_cast<unsigned>(static_cast<unsigned short>(static_cast<short>(q)));
-}
+// Mantissas of a block with exponent e: rint(clamp(v * 2^(15-e), +-32767)).
+// Clamping v to +-32767 * 2^(e-15) first is the same clamp (both scalings are
+// exact), and adding 1.5 * 2^23 in the fma rounds v * 2^(15-e) to an integer
+// (ulp 1, ties to even) whose two's-complement low 16 bits are the mantissa.
+// With e in [-112, 127] both 2^(15-e) and the limit are normal floats.
 __device__ __forceinline__ uint2 float4_to_bfp16x4(float4 v, int e) {
-    return make_uint2(bfp16_mantissa(v.x, e) | (bfp16_mantissa(v.y, e) << 16),
-                      bfp16_mantissa(v.z, e) | (bfp16_mantissa(v.w, e) << 16));
+    const float s = __builtin_ldexpf(1.0f, 15 - e);
+    const float lim = __builtin_ldexpf(32767.0f, e - 15);
+    auto q = [s, lim](float x) {
+        return __float_as_uint(__builtin_fmaf(__builtin_amdgcn_fmed3f(x, -lim, lim), s, 12582912.0f));
+    };
+    return make_uint2(__builtin_amdgcn_perm(q(v.y), q(v.x), 0x05040100u),
+                      __builtin_amdgcn_perm(q(v.w), q(v.z), 0x05040100u));
 }
-// Shared exponent of a block whose max |v| is m: frexp exponent (m < 2^e),
-// clamped to int8; 0 for an all-zero block.
+// Shared exponent of a block whose max |v| is m >= 0: frexp exponent
+// (m < 2^e; v_frexp_exp gives 0 for m == 0) clamped to [-112, 127].
 __device__ __forceinline__ int bfp16_exponent(float m) {
-    if (!(m > 0.f)) return 0;
-    int e = __builtin_amdgcn_frexp_expf(m);
-    return e < -127 ? -127 : (e > 127 ? 127 : e);
+    const int e = __builtin_amdgcn_frexp_expf(m);
+    return e < -112 ? -112 : (e > 127 ? 127 : e);
 }
 __device__ __forceinline__ float max_abs4(float4 v) {
     return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));

@@ -46,7 +46,7 @@ bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
 // Whether (txv, ty) is a tile shape the library is built with.
 bool zq_shape_valid(int txv, int ty);
 // Fills unset fields of a shape.
-ZqShape zq_resolve(const Dims& d, ZqShape s);
+ZqShape zq_resolve(const Dims& d, ZqShape s, int kind);
 
 // Autotuner cache (tune.cpp): the fastest shape measured for a grid size;
 // kind 0 = wave step, 1 = heat step.
@@ -71,11 +71,12 @@ struct Field16 {
     int8_t* exp;
 };
 // The acoustic step on 16-bit fields (fmt: 1 bf16, 2 scaled fp16, 3 block
-// fp16), FP32 arithmetic in the reference association.  p_scale applies to
-// p_cur / p_prev / p_next, v_scale to vel2 (powers of two; fp16 only).
+// fp16), FP32 arithmetic in the reference association (exact) or the
+// fused-weight form.  p_scale applies to p_cur / p_prev / p_next, v_scale to
+// vel2 (powers of two; fp16 only).
 hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Field16& vel2,
                       const Field16& next, const Dims& d, const Scalars<float>& s, float p_scale,
-                      float v_scale, ZqShape shape, hipStream_t st);
+                      float v_scale, bool exact, ZqShape shape, hipStream_t st);
 // Elementwise conversion between FP32 and a 16-bit format (n % 16 == 0).
 hipError_t encode16(int fmt, const float* src, Field16 dst, size_t n, float scale, hipStream_t st);
 hipError_t decode16(int fmt, Field16 src, float* dst, size_t n, float scale, hipStream_t st);

@@ -82,23 +82,28 @@ float time_launches(F&& fn, int reps, hipStream_t st) {
 }
 
 std::vector<ZqShape> candidates(const Dims& d) {
-    static const int shapes[][2] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}};
+    static const int shapes[][2] = {{16, 16}, {16, 32}, {32, 8}, {32, 16},
+                                    {64, 4},  {64, 8},  {64, 12}, {64, 16}};
     const int64_t inz = d.nz - 8;
     std::vector<int> chunks;
     for (int c : {1, 2, 4, 8, 16})
         if (c == 1 || inz / c >= 16) chunks.push_back(c);
     std::vector<ZqShape> out;
+    // xcd_remap 0 / 1: workgroup order as dispatched, or each XCD a
+    // contiguous range of tiles (1 needs a workgroup count divisible by 8;
+    // the launcher falls back to 0 otherwise, so such rows repeat 0).
     for (const auto& sh : shapes)
         for (int c : chunks)
-            for (int nt : {1, 0}) {
-                ZqShape s;
-                s.txv = sh[0];
-                s.ty = sh[1];
-                s.n_chunks = c;
-                s.xcd_remap = -1;
-                s.nt = nt;
-                out.push_back(s);
-            }
+            for (int nt : {1, 0})
+                for (int remap : {1, 0}) {
+                    ZqShape s;
+                    s.txv = sh[0];
+                    s.ty = sh[1];
+                    s.n_chunks = c;
+                    s.xcd_remap = remap;
+                    s.nt = nt;
+                    out.push_back(s);
+                }
     return out;
 }
 
@@ -119,11 +124,11 @@ int finish(const Dims& d, int kind, const std::vector<Row>& rows, float naive_ms
     if (csv_path && *csv_path) {
         FILE* f = std::fopen(csv_path, "w");
         if (!f) return STENCIL_ERR_INVALID;
-        std::fprintf(f, "nx,ny,nz,tile_x,tile_y,n_chunks,nontemporal,naive_ms,tiled_ms,speedup\n");
+        std::fprintf(f, "nx,ny,nz,tile_x,tile_y,n_chunks,nontemporal,xcd_remap,naive_ms,tiled_ms,speedup\n");
         for (const auto& r : rows)
-            std::fprintf(f, "%lld,%lld,%lld,%d,%d,%d,%d,%.5f,%.5f,%.3f\n", (long long)d.nx,
+            std::fprintf(f, "%lld,%lld,%lld,%d,%d,%d,%d,%d,%.5f,%.5f,%.3f\n", (long long)d.nx,
                          (long long)d.ny, (long long)d.nz, 4 * r.shape.txv, r.shape.ty,
-                         r.shape.n_chunks, r.shape.nt, naive_ms, r.ms,
+                         r.shape.n_chunks, r.shape.nt, r.shape.xcd_remap, naive_ms, r.ms,
                          r.ms > 0.f ? naive_ms / r.ms : 0.f);
         std::fclose(f);
     }
@@ -188,15 +193,46 @@ int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* gri
     return finish(d, 1, rows, naive_ms, best, csv_path);
 }
 
+int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stencil_field16* p_prev,
+                          const stencil_field16* vel2, const stencil_field16* p_next,
+                          const stencil_grid* grid, float p_scale, float v_scale, int reps,
+                          stencil_tile_shape* best, const char* csv_path, void* stream) {
+    if (storage < STENCIL_STORE_BF16 || storage > STENCIL_STORE_BFP16 || !dims_ok(grid) || reps <= 0)
+        return STENCIL_ERR_INVALID;
+    const stencil_field16* f[4] = {p_cur, p_prev, vel2, p_next};
+    for (const auto* x : f)
+        if (!x || !x->data || (storage == STENCIL_STORE_BFP16) != (x->exp != nullptr))
+            return STENCIL_ERR_INVALID;
+    if (p_next->data == p_cur->data || p_next->data == p_prev->data || p_next->data == vel2->data)
+        return STENCIL_ERR_INVALID;
+    Dims d;
+    d.nx = int64_t(grid->nx);
+    d.ny = int64_t(grid->ny);
+    d.nz = int64_t(grid->nz);
+    if (d.nx % 16 != 0 || d.nx * d.ny * 2 >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
+    const auto s = make_scalars<float>(float(grid->dx), float(grid->dy), float(grid->dz),
+                                       float(grid->dt));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const Field16 c{p_cur->data, p_cur->exp}, p{p_prev->data, p_prev->exp};
+    const Field16 v{vel2->data, vel2->exp}, n{p_next->data, p_next->exp};
+    std::vector<Row> rows;
+    for (const ZqShape& sh : candidates(d)) {
+        const float ms = time_launches(
+            [&] { return wave_zq_16(storage, c, p, v, n, d, s, p_scale, v_scale, false, sh, st); }, reps, st);
+        rows.push_back({sh, ms});
+    }
+    return finish(d, 2 + storage, rows, 0.f, best, csv_path);
+}
+
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out) {
-    if (!dims_ok(grid) || !out || kind < 0 || kind > 1) return STENCIL_ERR_INVALID;
+    if (!dims_ok(grid) || !out || kind < 0 || kind > 2 + STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
     Dims d;
     d.nx = int64_t(grid->nx);
     d.ny = int64_t(grid->ny);
     d.nz = int64_t(grid->nz);
     ZqShape s;
     const bool hit = zq_cached_shape(d, kind, &s);
-    if (!hit) s = zq_resolve(d, ZqShape{});
+    if (!hit) s = zq_resolve(d, ZqShape{}, kind);
     out->tile_x4 = s.txv;
     out->tile_y = s.ty;
     out->n_chunks = s.n_chunks;

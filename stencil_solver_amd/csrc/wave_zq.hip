@@ -22,6 +22,8 @@
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
 // write p_next once = 16 B (halo re-reads are L2 hits; the tile order keeps
 // neighbouring tiles on one XCD).
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "launchers.hpp"
 
@@ -55,59 +57,107 @@ struct ZqArgs {
 
 using dev::Fmt;
 
-// Loads the 4 values at element index idx (a multiple of 4) of a field in
-// storage format FMT and returns them as float (decoding is exact).
+// Raw storage of the 4 values one thread owns in one plane.
+template <int FMT>
+using Raw = typename std::conditional<FMT == dev::kF32, float4, uint2>::type;
+
+// Every field access goes through a raw buffer resource for one z-plane
+// (base = the plane's first byte, built in SGPRs; range = the plane's bytes):
+// plane addressing is scalar work, each lane supplies a 32-bit byte offset,
+// and an offset outside the plane reads 0 / drops the write instead of
+// touching other memory.  0x00020000 is the gfx9 word 3 (32-bit data format).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc plane_rsrc(const void* base, int z, size_t bytes) {
+    char* p = const_cast<char*>(static_cast<const char*>(base)) + static_cast<size_t>(z) * bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(static_cast<unsigned>(bytes)), 0x00020000);
+}
+// Cache policy bits of gfx950 buffer accesses: 2 = non-temporal.
+constexpr int kNtAux = 2;
+
+// Loads the 4 stored values at byte offset boff of a plane.
 template <int FMT, bool NT>
-__device__ __forceinline__ float4 load4(const void* base, const int8_t* ebase, size_t idx, float inv) {
+__device__ __forceinline__ Raw<FMT> load_raw(Rsrc r, unsigned boff) {
+    constexpr int aux = NT ? kNtAux : 0;
     if constexpr (FMT == dev::kF32) {
-        const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(base) + idx);
-        if constexpr (NT) return dev::load_stream(p);
-        return *p;
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(boff), 0, aux);
+        return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                           __uint_as_float(v.w));
     } else {
-        const uint2* p = reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(base) + idx);
-        const uint2 u = NT ? dev::load_stream_u2(p) : *p;
-        if constexpr (FMT == dev::kBf16) return dev::bf16x4_to_float4(u);
-        if constexpr (FMT == dev::kFp16s) return dev::fp16x4_to_float4(u, inv);
-        return dev::bfp16x4_to_float4(u, ebase[idx >> 4]);
+        const dev::v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(boff), 0, aux);
+        return make_uint2(v.x, v.y);
     }
 }
 
-// Stores 4 values at element index idx.  Every lane of a wave reaches this
-// (kBfp16 shares one exponent across a quad of lanes); `vals` says whether this
-// lane writes its values, `blk` whether its quad's block holds outputs.
-template <int FMT, bool NT>
-__device__ __forceinline__ void store4(void* base, int8_t* ebase, size_t idx, float4 v, float scale,
-                                       bool vals, bool blk) {
+// Exact decode of 4 stored values (e: the block exponent, kBfp16 only).
+// With SCALED false a kFp16s field is returned as stored (value * scale): the
+// fused forms run in the scaled domain, which moves every intermediate by the
+// same power of two.
+template <int FMT, bool SCALED>
+__device__ __forceinline__ float4 decode(Raw<FMT> u, int e, float inv) {
+    if constexpr (FMT == dev::kF32) return u;
+    else if constexpr (FMT == dev::kBf16) return dev::bf16x4_to_float4(u);
+    else if constexpr (FMT == dev::kFp16s) return SCALED ? dev::fp16x4_to_float4(u, inv) : dev::fp16x4_raw(u);
+    else return dev::bfp16x4_to_float4(u, e);
+}
+
+// Stores 4 values at byte offset boff of a plane.  Every lane of a wave
+// reaches this (kBfp16 shares one exponent across a quad of lanes); `vals`
+// says whether this lane writes its values, `blk` whether its quad's block
+// holds outputs (then lane 0 of the quad writes the exponent at byte eoff of
+// the plane's exponent row `er`).
+template <int FMT, bool NT, bool SCALED>
+__device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned eoff, float4 v,
+                                       float scale, bool vals, bool blk) {
+    constexpr int aux = NT ? kNtAux : 0;
     if constexpr (FMT == dev::kF32) {
         if (vals) {
-            float4* p = reinterpret_cast<float4*>(static_cast<float*>(base) + idx);
-            if constexpr (NT) dev::store_stream(p, v);
-            else *p = v;
+            typedef unsigned v4u __attribute__((ext_vector_type(4)));
+            const v4u w = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                           __float_as_uint(v.w)};
+            __builtin_amdgcn_raw_buffer_store_b128(w, r, static_cast<int>(boff), 0, aux);
         }
-    } else if constexpr (FMT == dev::kBfp16) {
-        const float m = vals ? dev::max_abs4(v) : 0.f;
-        const int e = dev::bfp16_exponent(dev::quad_max(m));
-        uint2* p = reinterpret_cast<uint2*>(static_cast<uint16_t*>(base) + idx);
-        const uint2 u = dev::float4_to_bfp16x4(v, e);
-        if (vals) {
-            if constexpr (NT) dev::store_stream_u2(p, u);
-            else *p = u;
-        }
-        if (blk && (threadIdx.x & 3) == 0) ebase[idx >> 4] = static_cast<int8_t>(e);
     } else {
-        if (vals) {
-            const uint2 u = FMT == dev::kBf16 ? dev::float4_to_bf16x4(v) : dev::float4_to_fp16x4(v, scale);
-            uint2* p = reinterpret_cast<uint2*>(static_cast<uint16_t*>(base) + idx);
-            if constexpr (NT) dev::store_stream_u2(p, u);
-            else *p = u;
+        uint2 u;
+        int e = 0;
+        if constexpr (FMT == dev::kBfp16) {
+            const float m = vals ? dev::max_abs4(v) : 0.f;
+            e = dev::bfp16_exponent(dev::quad_max(m));
+            u = dev::float4_to_bfp16x4(v, e);
+        } else if constexpr (FMT == dev::kBf16) {
+            u = dev::float4_to_bf16x4(v);
+        } else {
+            u = SCALED ? dev::float4_to_fp16x4(v, scale) : dev::fp16x4_pack(v);
         }
+        if (vals) {
+            const dev::v2u w = {u.x, u.y};
+            __builtin_amdgcn_raw_buffer_store_b64(w, r, static_cast<int>(boff), 0, aux);
+        }
+        if constexpr (FMT == dev::kBfp16)
+            if (blk && (threadIdx.x & 3) == 0)
+                __builtin_amdgcn_raw_buffer_store_b8(static_cast<unsigned char>(e), er, static_cast<int>(eoff), 0, 0);
     }
+}
+
+// Waves per SIMD the register budget is sized for: 4 (128 VGPRs: two
+// 512-thread or four 256-thread workgroups per CU, one 1024-thread one)
+// where the kernel fits without scratch; 3 (168 VGPRs) for a 768-thread
+// workgroup, and where 128 would spill: three halo items per thread (64x4),
+// or two with block-FP16 storage (exponent registers).
+constexpr int min_waves(int txv, int ty, int fmt) {
+    const int threads = txv * ty;
+    const int halo_per = (2 * dev::kR * txv + 2 * ty + threads - 1) / threads;
+    if (threads == 768 || halo_per >= 3 || (halo_per == 2 && fmt == dev::kBfp16)) return 3;
+    return 4;
 }
 
 template <int TXV, int TY, int MODE, bool NT, int FMT = dev::kF32>
-__global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
+__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, FMT))))
+void zq_kernel(ZqArgs a) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
     constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact);
+    constexpr bool kBlk = (FMT == dev::kBfp16);
+    constexpr unsigned kEsz = FMT == dev::kF32 ? 4u : 2u;  // bytes per stored value
     constexpr int kThreads = TXV * TY;
     constexpr int kCols = TXV + 2;  // own float4 + left/right halo float4
     constexpr int kRows = TY + 2 * kR;
@@ -133,10 +183,13 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
     if (z_begin >= z_end) return;  // uniform per workgroup
 
     const size_t plane = static_cast<size_t>(a.nx) * static_cast<size_t>(a.ny);
+    const size_t plane_bytes = plane * kEsz;
+    const size_t eplane = plane >> 4;  // block exponents per plane (nx % 16 == 0)
     const bool own_in = (x < a.nx) && (y < a.ny);
     const bool own_out = (x >= kR) && (x < a.nx - kR) && (y < a.ny - kR);
     const unsigned own_off = static_cast<unsigned>(y) * static_cast<unsigned>(a.nx) +
                              static_cast<unsigned>(x);
+    const unsigned own_b = own_off * kEsz;
 
     // This thread's halo float4s: item i -> (row, col) of the LDS tile.
     int h_row[kHaloPer], h_col[kHaloPer];
@@ -169,128 +222,204 @@ __global__ __launch_bounds__(TXV* TY) void zq_kernel(ZqArgs a) {
                            : 0u;
     }
 
-    // Predicated loads; the zero is built in registers so the compiler cannot
-    // turn the predicate into a select between a global and a private address.
-    const void* const cur_base = a.cur;
-    const int8_t* const cur_e = a.cur_e;
-    const float p_inv = a.p_inv;
-    auto cur_at = [cur_base, cur_e, p_inv, plane](int z, unsigned off, bool ok) -> float4 {
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok) v = load4<FMT, false>(cur_base, cur_e, static_cast<size_t>(z) * plane + off, p_inv);
-        return v;
+    const char* const cur_b = static_cast<const char*>(a.cur);
+    const char* const prev_b = static_cast<const char*>(a.prev);
+    const char* const vel_b = static_cast<const char*>(a.vel2);
+    char* const next_b = static_cast<char*>(a.next);
+    const float p_inv = a.p_inv, v_inv = a.v_inv;
+
+    // Loads are unconditional.  A lane whose cell lies outside the grid (x or y
+    // past the last row / column, the x < 0 halo of the first tile column, the
+    // unused halo items) reads offset 0 instead: the LDS cell or register it
+    // fills is never read by a lane that stores an output (an output lane at
+    // x reads x-4..x+7 and y-4..y+4, all inside the grid).
+    const unsigned own_ld = own_in ? own_b : 0u;
+    const unsigned own_eld = own_in ? (own_off >> 4) : 0u;
+    unsigned h_ld[kHaloPer], h_eld[kHaloPer];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+        h_ld[k] = h_in[k] ? h_off[k] * kEsz : 0u;
+        h_eld[k] = h_in[k] ? (h_off[k] >> 4) : 0u;
+    }
+    const int z_last = a.nz - 1;
+    auto raw_at = [plane_bytes](const char* f, int z, unsigned boff) -> Raw<FMT> {
+        return load_raw<FMT, false>(plane_rsrc(f, z, plane_bytes), boff);
     };
-    auto stream_at = [plane, own_off, own_out](const void* f, const int8_t* fe, float inv, int z) -> float4 {
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own_out) v = load4<FMT, NT>(f, fe, static_cast<size_t>(z) * plane + own_off, inv);
-        return v;
+    auto raw_stream = [plane_bytes, own_ld](const char* f, int z) -> Raw<FMT> {
+        return load_raw<FMT, NT>(plane_rsrc(f, z, plane_bytes), own_ld);
+    };
+    auto exp_at = [eplane](const int8_t* e, int z, unsigned eoff) -> int {
+        if constexpr (kBlk)
+            return static_cast<signed char>(
+                __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(e, z, eplane), static_cast<int>(eoff), 0, 0));
+        return 0;
     };
     // This lane's quad (16 x-consecutive cells) holds output cells: it owns
     // the block exponent of kBfp16 storage.
     const int xq = x & ~15;
     const bool blk_out = (y >= kR) && (y < a.ny - kR) && (xq < a.nx - kR);
 
+    // One plane of the star stencil for this thread's 4 points: LDS buffer b
+    // holds plane zz (x and y neighbours), zq[k] is plane zz - 4 + k.
+    auto compute = [&](int b, const float4 (&zq)[9], float4 pv, float4 vv) -> float4 {
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 ctr4 = zq[kR];
+        const float4 xl = lds[b][ty + kR][tx];
+        const float4 xr = lds[b][ty + kR][tx + 2];
+        const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
+                             ctr4.z, ctr4.w, xr.x,   xr.y,   xr.z,   xr.w};
+        float4 ym[kR], yp[kR];
+#pragma unroll
+        for (int r = 1; r <= kR; ++r) {
+            ym[r - 1] = lds[b][ty + kR - r][tx + 1];
+            yp[r - 1] = lds[b][ty + kR + r][tx + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float p = w[4 + i];
+            float res;
+            if constexpr (kExact) {
+                // src/kernels/stencil_naive.cu:75-88, op for op.
+                float lx = Fd<float>::c0 * p * a.s.inv_dx2;
+                float ly = Fd<float>::c0 * p * a.s.inv_dy2;
+                float lz = Fd<float>::c0 * p * a.s.inv_dz2;
+#pragma unroll
+                for (int r = 1; r <= kR; ++r) {
+                    lx += Fd<float>::c(r) * (w[4 + i + r] + w[4 + i - r]) * a.s.inv_dx2;
+                    ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) * a.s.inv_dy2;
+                    lz += Fd<float>::c(r) * (comp(zq[kR + r], i) + comp(zq[kR - r], i)) * a.s.inv_dz2;
+                }
+                if constexpr (kHeat) res = p + a.coef * (lx + ly + lz);
+                else res = 2.f * p - comp(pv, i) + a.s.dt2 * comp(vv, i) * (lx + ly + lz);
+            } else {
+                // Combined weights w_r = c_r / h^2, one accumulator.
+                float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
+#pragma unroll
+                for (int r = 1; r <= kR; ++r) {
+                    acc = __builtin_fmaf(a.wx[r - 1], w[4 + i + r] + w[4 + i - r], acc);
+                    acc = __builtin_fmaf(a.wy[r - 1], comp(yp[r - 1], i) + comp(ym[r - 1], i), acc);
+                    acc = __builtin_fmaf(a.wz[r - 1], comp(zq[kR + r], i) + comp(zq[kR - r], i), acc);
+                }
+                if constexpr (kHeat) {
+                    res = acc;
+                } else {
+                    const float base = __builtin_fmaf(2.f, p, -comp(pv, i));
+                    res = __builtin_fmaf(a.s.dt2 * comp(vv, i), acc, base);
+                }
+            }
+            dev::set_comp(out, i, res);
+        }
+        return out;
+    };
+    // Steps past z_end (the tail of the last group of 9 in a chunk whose length
+    // is not a multiple of 9) run on clamped planes and store nothing.
+    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 pv, float4 vv) {
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (own_out) out = compute(b, zq, pv, vv);
+        if (zz < z_end)
+            store4<FMT, NT, kExact>(plane_rsrc(next_b, zz, plane_bytes),
+                                    plane_rsrc(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
+                                    a.p_scale, own_out, blk_out);
+    };
+
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
     float4 q[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) q[k] = cur_at(z_begin - kR + k, own_off, own_in);
-    float4 h_cur[kHaloPer];
-#pragma unroll
-    for (int k = 0; k < kHaloPer; ++k) h_cur[k] = cur_at(z_begin, h_off[k], h_in[k]);
-    float4 pv_cur = make_float4(0.f, 0.f, 0.f, 0.f), vv_cur = pv_cur;
-    if constexpr (!kHeat) {
-        pv_cur = stream_at(a.prev, a.prev_e, a.p_inv, z_begin);
-        vv_cur = stream_at(a.vel2, a.vel2_e, a.v_inv, z_begin);
-    }
+    for (int k = 0; k < 8; ++k)
+        q[k] = decode<FMT, kExact>(raw_at(cur_b, z_begin - kR + k, own_ld),
+                                   exp_at(a.cur_e, z_begin - kR + k, own_eld), p_inv);
 
-    for (int zb = z_begin; zb < z_end; zb += 9) {
+    if constexpr (FMT == dev::kF32) {
+        // FP32: the queue refill (plane zz+4) is loaded at the top of step zz;
+        // halo and streams for plane zz+1 are prefetched one step ahead.
+        float4 h_cur[kHaloPer];
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const int zz = zb + j;
-            if (zz < z_end) {  // uniform per workgroup; a constant trip count keeps the unroll
-                // Refill the queue (plane zz+4) and prefetch plane zz+1's halo
-                // and streams; all in bounds since zz+4 <= nz-1.
-                q[(j + 8) % 9] = cur_at(zz + kR, own_off, own_in);
+        for (int k = 0; k < kHaloPer; ++k) h_cur[k] = raw_at(cur_b, z_begin, h_ld[k]);
+        float4 pv_cur = make_float4(0.f, 0.f, 0.f, 0.f), vv_cur = pv_cur;
+        if constexpr (!kHeat) {
+            pv_cur = raw_stream(prev_b, z_begin);
+            vv_cur = raw_stream(vel_b, z_begin);
+        }
+        for (int zb = z_begin; zb < z_end; zb += 9) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int zz = zb + j;
+                const int z1 = min(zz + 1, z_last);
+                q[(j + 8) % 9] = raw_at(cur_b, min(zz + kR, z_last), own_ld);
                 float4 h_nxt[kHaloPer];
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) h_nxt[k] = cur_at(zz + 1, h_off[k], h_in[k]);
+                for (int k = 0; k < kHaloPer; ++k) h_nxt[k] = raw_at(cur_b, z1, h_ld[k]);
                 float4 pv_nxt = make_float4(0.f, 0.f, 0.f, 0.f), vv_nxt = pv_nxt;
                 if constexpr (!kHeat) {
-                    pv_nxt = stream_at(a.prev, a.prev_e, a.p_inv, zz + 1);
-                    vv_nxt = stream_at(a.vel2, a.vel2_e, a.v_inv, zz + 1);
+                    pv_nxt = raw_stream(prev_b, z1);
+                    vv_nxt = raw_stream(vel_b, z1);
                 }
-
                 const int b = (zz - z_begin) & 1;
-                const float4 ctr4 = q[(j + 4) % 9];
-                lds[b][ty + kR][tx + 1] = ctr4;
+                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
                     if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h_cur[k];
                 __syncthreads();
-
-                float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (own_out) {
-                    const float4 xl = lds[b][ty + kR][tx];
-                    const float4 xr = lds[b][ty + kR][tx + 2];
-                    const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
-                                         ctr4.z, ctr4.w, xr.x,   xr.y,   xr.z,   xr.w};
-                    float4 ym[kR], yp[kR];
-#pragma unroll
-                    for (int r = 1; r <= kR; ++r) {
-                        ym[r - 1] = lds[b][ty + kR - r][tx + 1];
-                        yp[r - 1] = lds[b][ty + kR + r][tx + 1];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float p = w[4 + i];
-                        float res;
-                        if constexpr (kExact) {
-                            // src/kernels/stencil_naive.cu:75-88, op for op.
-                            float lx = Fd<float>::c0 * p * a.s.inv_dx2;
-                            float ly = Fd<float>::c0 * p * a.s.inv_dy2;
-                            float lz = Fd<float>::c0 * p * a.s.inv_dz2;
-#pragma unroll
-                            for (int r = 1; r <= kR; ++r) {
-                                lx += Fd<float>::c(r) * (w[4 + i + r] + w[4 + i - r]) * a.s.inv_dx2;
-                                ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) *
-                                      a.s.inv_dy2;
-                                lz += Fd<float>::c(r) *
-                                      (comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i)) *
-                                      a.s.inv_dz2;
-                            }
-                            if constexpr (kHeat) {
-                                res = p + a.coef * (lx + ly + lz);
-                            } else {
-                                res = 2.f * p - comp(pv_cur, i) +
-                                      a.s.dt2 * comp(vv_cur, i) * (lx + ly + lz);
-                            }
-                        } else {
-                            // Combined weights w_r = c_r / h^2, one accumulator.
-                            float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
-#pragma unroll
-                            for (int r = 1; r <= kR; ++r) {
-                                acc = __builtin_fmaf(a.wx[r - 1], w[4 + i + r] + w[4 + i - r], acc);
-                                acc = __builtin_fmaf(a.wy[r - 1],
-                                                     comp(yp[r - 1], i) + comp(ym[r - 1], i), acc);
-                                acc = __builtin_fmaf(
-                                    a.wz[r - 1],
-                                    comp(q[(j + 4 + r) % 9], i) + comp(q[(j + 4 - r) % 9], i), acc);
-                            }
-                            if constexpr (kHeat) {
-                                res = acc;
-                            } else {
-                                const float base = __builtin_fmaf(2.f, p, -comp(pv_cur, i));
-                                res = __builtin_fmaf(a.s.dt2 * comp(vv_cur, i), acc, base);
-                            }
-                        }
-                        dev::set_comp(out, i, res);
-                    }
-                }
-                store4<FMT, NT>(a.next, a.next_e, static_cast<size_t>(zz) * plane + own_off, out,
-                                a.p_scale, own_out, blk_out);
+                const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
+                                      q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
+                                      q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
+                finish(zz, b, zq, pv_cur, vv_cur);
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k) h_cur[k] = h_nxt[k];
                 pv_cur = pv_nxt;
                 vv_cur = vv_nxt;
+            }
+        }
+    } else {
+        // 16-bit storage: every load is issued one step ahead as raw 16-bit
+        // words (the queue refill for plane zz+5, halo and streams for plane
+        // zz+1) and decoded at the top of the step that uses it, so the raw
+        // registers are free again before they are reloaded.
+        Raw<FMT> rq = raw_at(cur_b, z_begin + kR, own_ld);
+        int rq_e = exp_at(a.cur_e, z_begin + kR, own_eld);
+        Raw<FMT> rh[kHaloPer];
+        int rh_e[kHaloPer];
+#pragma unroll
+        for (int k = 0; k < kHaloPer; ++k) {
+            rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
+            rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
+        }
+        Raw<FMT> rp = raw_stream(prev_b, z_begin), rv = raw_stream(vel_b, z_begin);
+        int rp_e = exp_at(a.prev_e, z_begin, own_eld);
+        int rv_e = exp_at(a.vel2_e, z_begin, own_eld);
+        for (int zb = z_begin; zb < z_end; zb += 9) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int zz = zb + j;
+                const int z1 = min(zz + 1, z_last);
+                const int z5 = min(zz + kR + 1, z_last);
+                q[(j + 8) % 9] = decode<FMT, kExact>(rq, rq_e, p_inv);
+                float4 h[kHaloPer];
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k) h[k] = decode<FMT, kExact>(rh[k], rh_e[k], p_inv);
+                const float4 pv = decode<FMT, kExact>(rp, rp_e, p_inv);
+                const float4 vv = decode<FMT, kExact>(rv, rv_e, v_inv);
+                rq = raw_at(cur_b, z5, own_ld);
+                rq_e = exp_at(a.cur_e, z5, own_eld);
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k) {
+                    rh[k] = raw_at(cur_b, z1, h_ld[k]);
+                    rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
+                }
+                rp = raw_stream(prev_b, z1);
+                rv = raw_stream(vel_b, z1);
+                rp_e = exp_at(a.prev_e, z1, own_eld);
+                rv_e = exp_at(a.vel2_e, z1, own_eld);
+                const int b = (zz - z_begin) & 1;
+                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h[k];
+                __syncthreads();
+                const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
+                                      q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
+                                      q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
+                finish(zz, b, zq, pv, vv);
             }
         }
     }
@@ -302,7 +431,7 @@ using KernelFn = void (*)(ZqArgs);
 struct TileShape {
     int txv, ty;
 };
-constexpr TileShape kShapes[] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}};
+constexpr TileShape kShapes[] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}, {64, 12}, {64, 16}};
 
 template <int MODE, bool NT>
 KernelFn pick_shape(int txv, int ty) {
@@ -312,6 +441,8 @@ KernelFn pick_shape(int txv, int ty) {
     if (txv == 32 && ty == 16) return zq_kernel<32, 16, MODE, NT>;
     if (txv == 64 && ty == 4) return zq_kernel<64, 4, MODE, NT>;
     if (txv == 64 && ty == 8) return zq_kernel<64, 8, MODE, NT>;
+    if (txv == 64 && ty == 12) return zq_kernel<64, 12, MODE, NT>;
+    if (txv == 64 && ty == 16) return zq_kernel<64, 16, MODE, NT>;
     return nullptr;
 }
 
@@ -324,24 +455,35 @@ KernelFn pick(int txv, int ty, int mode, bool nt) {
     }
 }
 
-template <int FMT, bool NT>
+template <int MODE, int FMT, bool NT>
 KernelFn pick_shape16(int txv, int ty) {
-    if (txv == 16 && ty == 16) return zq_kernel<16, 16, kWaveExact, NT, FMT>;
-    if (txv == 16 && ty == 32) return zq_kernel<16, 32, kWaveExact, NT, FMT>;
-    if (txv == 32 && ty == 8) return zq_kernel<32, 8, kWaveExact, NT, FMT>;
-    if (txv == 32 && ty == 16) return zq_kernel<32, 16, kWaveExact, NT, FMT>;
-    if (txv == 64 && ty == 4) return zq_kernel<64, 4, kWaveExact, NT, FMT>;
-    if (txv == 64 && ty == 8) return zq_kernel<64, 8, kWaveExact, NT, FMT>;
+    if (txv == 16 && ty == 16) return zq_kernel<16, 16, MODE, NT, FMT>;
+    if (txv == 16 && ty == 32) return zq_kernel<16, 32, MODE, NT, FMT>;
+    if (txv == 32 && ty == 8) return zq_kernel<32, 8, MODE, NT, FMT>;
+    if (txv == 32 && ty == 16) return zq_kernel<32, 16, MODE, NT, FMT>;
+    if (txv == 64 && ty == 4) return zq_kernel<64, 4, MODE, NT, FMT>;
+    if (txv == 64 && ty == 8) return zq_kernel<64, 8, MODE, NT, FMT>;
+    if (txv == 64 && ty == 12) return zq_kernel<64, 12, MODE, NT, FMT>;
+    if (txv == 64 && ty == 16) return zq_kernel<64, 16, MODE, NT, FMT>;
     return nullptr;
 }
 
-KernelFn pick16(int txv, int ty, int fmt, bool nt) {
+template <int MODE>
+KernelFn pick16_mode(int txv, int ty, int fmt, bool nt) {
     switch (fmt) {
-        case dev::kBf16: return nt ? pick_shape16<dev::kBf16, true>(txv, ty) : pick_shape16<dev::kBf16, false>(txv, ty);
-        case dev::kFp16s: return nt ? pick_shape16<dev::kFp16s, true>(txv, ty) : pick_shape16<dev::kFp16s, false>(txv, ty);
-        case dev::kBfp16: return nt ? pick_shape16<dev::kBfp16, true>(txv, ty) : pick_shape16<dev::kBfp16, false>(txv, ty);
+        case dev::kBf16:
+            return nt ? pick_shape16<MODE, dev::kBf16, true>(txv, ty) : pick_shape16<MODE, dev::kBf16, false>(txv, ty);
+        case dev::kFp16s:
+            return nt ? pick_shape16<MODE, dev::kFp16s, true>(txv, ty) : pick_shape16<MODE, dev::kFp16s, false>(txv, ty);
+        case dev::kBfp16:
+            return nt ? pick_shape16<MODE, dev::kBfp16, true>(txv, ty) : pick_shape16<MODE, dev::kBfp16, false>(txv, ty);
         default: return nullptr;
     }
+}
+
+KernelFn pick16(int txv, int ty, int fmt, int mode, bool nt) {
+    return mode == kWaveExact ? pick16_mode<kWaveExact>(txv, ty, fmt, nt)
+                              : pick16_mode<kWaveFused>(txv, ty, fmt, nt);
 }
 
 // Fused weights: c_r / h^2 formed in double from the exact rationals and the
@@ -360,9 +502,9 @@ void fill_weights(ZqArgs& a, double scale) {
 hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st,
                      int fmt = dev::kF32) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
-    shape = zq_resolve(d, shape);
+    shape = zq_resolve(d, shape, fmt != dev::kF32 ? 2 + fmt : (mode == kHeatFused || mode == kHeatExact ? 1 : 0));
     KernelFn fn = fmt == dev::kF32 ? pick(shape.txv, shape.ty, mode, shape.nt != 0)
-                                   : pick16(shape.txv, shape.ty, fmt, shape.nt != 0);
+                                   : pick16(shape.txv, shape.ty, fmt, mode, shape.nt != 0);
     if (!fn) return hipErrorInvalidValue;
     const int64_t inz = d.nz - 2 * kR;
     a.nx = static_cast<int>(d.nx);
@@ -370,7 +512,9 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     a.nz = static_cast<int>(d.nz);
     a.tiles_x = static_cast<int>((d.nx + 4 * shape.txv - 1) / (4 * shape.txv));
     a.tiles_y = static_cast<int>((d.ny - 2 * kR + shape.ty - 1) / shape.ty);
-    a.chunk = static_cast<int>((inz + shape.n_chunks - 1) / shape.n_chunks);
+    // Chunks are whole groups of 9 planes (the kernel's unroll), so only the
+    // last chunk of a column runs idle steps.
+    a.chunk = static_cast<int>(((inz + shape.n_chunks - 1) / shape.n_chunks + 8) / 9 * 9);
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = (shape.xcd_remap != 0 && nwg % 8 == 0) ? 1 : 0;
@@ -382,8 +526,8 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
 
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs) {
     if (d.nx % 4 != 0) return false;
-    // 32-bit in-plane offsets.
-    if (d.nx * d.ny >= (int64_t(1) << 32)) return false;
+    // 32-bit in-plane byte offsets.
+    if (d.nx * d.ny * 4 >= (int64_t(1) << 32)) return false;
     for (int i = 0; i < n_ptrs; ++i)
         if (reinterpret_cast<uintptr_t>(ptrs[i]) % 16 != 0) return false;
     return true;
@@ -395,10 +539,12 @@ bool zq_shape_valid(int txv, int ty) {
     return false;
 }
 
-ZqShape zq_resolve(const Dims& d, ZqShape s) {
+ZqShape zq_resolve(const Dims& d, ZqShape s, int kind) {
     if (!zq_shape_valid(s.txv, s.ty)) {
         // Widest tile whose row padding past nx is smallest (ties go wide):
-        // 256-cell rows at nx = 512, 128 at nx = 128, ...; 512 threads each.
+        // 256-cell rows at nx = 512, 128 at nx = 128, ...  Rows: 16-bit storage
+        // and wide grids take 64 x 16 (one 1024-thread workgroup per CU),
+        // otherwise 512 threads per workgroup.
         int best = 16;
         int64_t best_waste = -1;
         for (int t : {64, 32, 16}) {
@@ -409,16 +555,18 @@ ZqShape zq_resolve(const Dims& d, ZqShape s) {
             }
         }
         s.txv = best;
-        s.ty = best == 64 ? 8 : best == 32 ? 16 : 32;
+        s.ty = best == 64 ? ((kind >= 2 || d.nx >= 1024) ? 16 : 8) : best == 32 ? 16 : 32;
     }
     if (s.nt < 0) s.nt = 1;
     const int64_t inz = d.nz - 2 * kR;
     if (s.n_chunks <= 0) {
-        // About two workgroups per CU (256 CUs), chunks of at least 32 planes:
-        // long z-marches stream best.
+        // z-chunks for about one workgroup per CU (256 CUs; two for the
+        // single-level heat kernel), chunks of at least 32 planes: long
+        // z-marches stream best.  These are the autotuner's picks at 512^3.
         const int64_t tiles =
             ((d.nx + 4 * s.txv - 1) / (4 * s.txv)) * ((d.ny - 2 * kR + s.ty - 1) / s.ty);
-        int64_t want = (512 + tiles / 2) / tiles;
+        const int64_t target = kind == 1 ? 512 : 256;
+        int64_t want = (target + tiles / 2) / tiles;
         const int64_t max_chunks = inz >= 32 ? inz / 32 : 1;
         if (want > max_chunks) want = max_chunks;
         if (want < 1) want = 1;
@@ -444,7 +592,7 @@ hipError_t wave_zq_f32(const float* cur, const float* prev, const float* vel2, f
 
 hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Field16& vel2,
                       const Field16& next, const Dims& d, const Scalars<float>& s, float p_scale,
-                      float v_scale, ZqShape shape, hipStream_t st) {
+                      float v_scale, bool exact, ZqShape shape, hipStream_t st) {
     ZqArgs a{};
     a.cur = cur.data;
     a.prev = prev.data;
@@ -458,7 +606,13 @@ hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Fi
     a.p_inv = 1.0f / p_scale;
     a.v_inv = 1.0f / v_scale;
     a.s = s;
-    return launch_zq(a, d, shape, kWaveExact, st, fmt);
+    if (!exact) {
+        // Fused forms run on the stored values: p in its scaled domain (the
+        // update is linear in p), the vel2 scale folded into dt^2.
+        if (fmt == dev::kFp16s) a.s.dt2 = s.dt2 * a.v_inv;
+        fill_weights(a, 1.0);
+    }
+    return launch_zq(a, d, shape, exact ? kWaveExact : kWaveFused, st, fmt);
 }
 
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,

@@ -341,7 +341,7 @@ def test_autotuner_picks_and_caches_shape(ss, dev, oracle, tmp_path):
     after, hit = ss.tuned_shape(grid)
     assert hit and after == best
     lines = csv.read_text().strip().splitlines()
-    assert lines[0].startswith("nx,ny,nz,tile_x,tile_y,n_chunks,nontemporal,naive_ms,tiled_ms,speedup")
+    assert lines[0].startswith("nx,ny,nz,tile_x,tile_y,n_chunks,nontemporal,xcd_remap,naive_ms,tiled_ms,speedup")
     assert len(lines) > 20
     # the tuned launch is still bit-identical to the oracle
     ref = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 6)

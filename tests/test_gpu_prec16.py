@@ -65,12 +65,81 @@ def test_step16_bit_exact_vs_oracle(ss, dev, oracle, fmt, shape, tile):
     i = 0
     for _ in range(5):
         oracle.wave_step16(fmt, o[(i + 1) % 3], o[i], ov, o[(i + 2) % 3], grid.dx, grid.dt, ps, vs, grid.dy, grid.dz)
-        ss.launch_tiled_16(g[(i + 1) % 3], g[i], gv, g[(i + 2) % 3], grid, tile=shp)
+        ss.launch_tiled_16(g[(i + 1) % 3], g[i], gv, g[(i + 2) % 3], grid, tile=shp, exact=True)
         i = (i + 1) % 3
     for k in range(3):
         assert np.array_equal(g[k].data.cpu().numpy().view(np.uint16), o[k][0]), k
         if fmt == "bfp16":
             assert np.array_equal(g[k].exp.cpu().numpy(), o[k][1]), k
+
+
+# Fused-weight form against the oracle's reference-association 16-bit step:
+# the FP32 results differ by a few ulps, so a stored value can land one 16-bit
+# step away where the FP32 value sits next to a rounding boundary.
+FUSED_TOL = {"bf16": 2.0 ** -8, "fp16": 2.0 ** -11, "bfp16": 2.0 ** -15}
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("shape", [(36, 52, 64), (20, 61, 48)])
+@pytest.mark.parametrize("tile", [None, (64, 8, 2), (16, 16, 3)])
+def test_step16_fused_vs_oracle(ss, dev, oracle, fmt, shape, tile):
+    p0, vel2, ps, vs = _fields(ss, oracle, fmt, shape, dev, 37)
+    if fmt == "fp16":
+        ps = 2.0 ** 3  # the fused form runs in the scaled domain: exercise it
+    nz, ny, nx = shape
+    grid = ss.GridDims(nx, ny, nz, np.float32(10.0), np.float32(12.0), np.float32(8.0), np.float32(8e-4))
+    o = [oracle.encode16(fmt, p0, ps), oracle.encode16(fmt, p0, ps), oracle.encode16(fmt, np.zeros_like(p0), ps)]
+    ov = oracle.encode16(fmt, vel2, vs)
+    g = [ss.encode16(fmt, to_dev(a, dev), ss.Field16(fmt, shape, dev, ps)) for a in (p0, p0, np.zeros_like(p0))]
+    gv = ss.encode16(fmt, to_dev(vel2, dev), ss.Field16(fmt, shape, dev, vs))
+    shp = ss.tile_shape(*tile) if tile else None
+    i = 0
+    for _ in range(4):
+        oracle.wave_step16(fmt, o[(i + 1) % 3], o[i], ov, o[(i + 2) % 3], grid.dx, grid.dt, ps, vs, grid.dy, grid.dz)
+        ss.launch_tiled_16(g[(i + 1) % 3], g[i], gv, g[(i + 2) % 3], grid, tile=shp)
+        i = (i + 1) % 3
+    got = ss.decode16(g[(i + 1) % 3]).cpu().numpy()
+    want = oracle.decode16(fmt, *o[(i + 1) % 3], scale=ps)
+    inner = (slice(R, -R),) * 3
+    peak = float(np.abs(want[inner]).max())
+    assert peak > 0.1
+    # relative to the field's peak: a few 16-bit steps at most, and rarely
+    err = np.abs(got[inner] - want[inner])
+    assert err.max() <= 4 * FUSED_TOL[fmt] * peak, err.max() / peak
+    assert np.mean(err > 0) < 0.05
+    halo = np.ones(shape, bool)
+    halo[inner] = False
+    assert np.all(got[halo] == 0)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_tune16_caches_a_shape_with_identical_results(ss, dev, oracle, fmt):
+    """The 16-bit autotuner caches its pick for NULL-shape launches on this
+    grid and format; a NULL-shape launch then equals the explicit shape."""
+    import torch
+
+    shape = (40, 44, 64)
+    p0, vel2, ps, vs = _fields(ss, oracle, fmt, shape, dev, 41)
+    nz, ny, nx = shape
+    grid = ss.GridDims(nx, ny, nz, np.float32(10.0), np.float32(10.0), np.float32(10.0), np.float32(1e-3))
+    ss.clear_tuning()
+    assert ss.tuned_shape(grid, fmt)[1] is False
+    g = [ss.encode16(fmt, to_dev(a, dev), ss.Field16(fmt, shape, dev, ps)) for a in (p0, p0 * 0.9)]
+    gv = ss.encode16(fmt, to_dev(vel2, dev), ss.Field16(fmt, shape, dev, vs))
+    out_t = ss.Field16(fmt, shape, dev, ps)
+    best = ss.tune_tiled_16(g[0], g[1], gv, out_t, grid, reps=1)
+    shp, hit = ss.tuned_shape(grid, fmt)
+    assert hit and shp == best
+    outs = []
+    for tile in (None, ss.tile_shape(best["tile_x"] // 4, best["tile_y"], best["n_chunks"],
+                                     best["xcd_remap"], best["nontemporal"])):
+        o = ss.encode16(fmt, torch.zeros(shape, device=dev), ss.Field16(fmt, shape, dev, ps))
+        ss.launch_tiled_16(g[0], g[1], gv, o, grid, tile=tile)
+        outs.append(o)
+    assert torch.equal(outs[0].data, outs[1].data)
+    if fmt == "bfp16":
+        assert torch.equal(outs[0].exp, outs[1].exp)
+    ss.clear_tuning()
 
 
 def test_reductions_match_numpy(ss, dev, oracle):

@@ -62,10 +62,14 @@ def test_bf16_round_to_nearest_even(oracle):
 
 
 def test_bfp16_block_format(oracle):
-    """Shared exponent = frexp exponent of the block max; each mantissa is the
-    nearest integer (ties to even) of v*2^(15-e), |m| <= 32767; decode exact."""
+    """Shared exponent = frexp exponent of the block max, clamped to
+    [-112, 127]; each mantissa is the nearest integer (ties to even) of
+    v*2^(15-e), |m| <= 32767; decode exact."""
     rng = np.random.default_rng(3)
-    blocks = [rng.standard_normal(16).astype(np.float32) * s for s in (1.0, 1e-3, 1e4, 1e-30)]
+    blocks = [rng.standard_normal(16).astype(np.float32) * s
+              for s in (1.0, 1e-3, 1e4, 1e-30, 1e-34, 1e-36, 1e-40)]
+    # a block max just below 2^e rounds up to 2^15: clamped to 32767
+    blocks.append(np.array([np.nextafter(np.float32(1.0), np.float32(0.0))] + [0.25] * 15, np.float32))
     blocks.append(np.zeros(16, np.float32))
     blocks.append(np.array([0.5] + [0.0] * 15, np.float32))
     blocks.append(np.array([-1.0] * 16, np.float32))
@@ -75,7 +79,7 @@ def test_bfp16_block_format(oracle):
     for b in range(len(blocks)):
         blk = x[16 * b:16 * b + 16].astype(np.float64)
         m = np.max(np.abs(blk))
-        want_e = 0 if m == 0 else int(np.frexp(m)[1])
+        want_e = 0 if m == 0 else min(127, max(-112, int(np.frexp(m)[1])))
         assert int(e[b]) == want_e
         for k in range(16):
             t = Fraction(float(blk[k])) * Fraction(2) ** (15 - want_e)
@@ -83,8 +87,11 @@ def test_bfp16_block_format(oracle):
             q = max(-32767, min(32767, q))
             assert int(d[16 * b + k].astype(np.int16)) == q
             assert Fraction(float(back[16 * b + k])) == Fraction(q) * Fraction(2) ** (want_e - 15)
-        if m > 0:
-            assert np.max(np.abs(back[16 * b:16 * b + 16] - blk)) <= 2.0 ** (want_e - 16) * (1 + 1e-12)
+        if m > 0:  # half a mantissa step, a whole one where 2^15 was clamped to 32767
+            err = np.abs(back[16 * b:16 * b + 16] - blk)
+            clamped = np.abs(blk) * 2.0 ** (15 - want_e) > 32767
+            assert np.all(err[~clamped] <= 2.0 ** (want_e - 16) * (1 + 1e-12))
+            assert np.all(err[clamped] <= 2.0 ** (want_e - 15))
 
 
 def test_fp16_scaled_roundtrip(oracle):
