@@ -373,30 +373,35 @@ float oracle_fp16_to_f32(uint16_t h) {
     return u2f(sign | ((e - 15 + 127) << 23) | (m << 13));
 }
 
+/* Block scale k of a block whose max |v| is m: e = frexp exponent of
+ * fl(m * (1 + 2^-15)), clamped to [-112, 127] (0 for m == 0), k = e - 15.
+ * The 1 + 2^-15 factor moves a max within 2^-16 of 2^e up to exponent e+1, so
+ * |v| * 2^-k < 32767.5 and the rounded mantissa never leaves [-32767, 32767]. */
 static int bfp16_exponent(float m) {
-    if (!(m > 0.0f)) return 0;
+    if (!(m > 0.0f)) return -15;
     int e;
-    frexpf(m, &e);
-    return e < -112 ? -112 : (e > 127 ? 127 : e);
+    frexpf(m * (1.0f + 0x1p-15f), &e);
+    e = e < -112 ? -112 : (e > 127 ? 127 : e);
+    return e - 15;
 }
 
-static uint16_t bfp16_mantissa(float v, int e) {
-    float q = rintf(ldexpf(v, 15 - e));
-    if (q > 32767.0f) q = 32767.0f;
+static uint16_t bfp16_mantissa(float v, int k) {
+    float q = rintf(ldexpf(v, -k));
+    if (q > 32767.0f) q = 32767.0f; /* unreachable by the choice of k; kept as a guard */
     if (q < -32767.0f) q = -32767.0f;
     return (uint16_t)(int16_t)q;
 }
 
-static float bfp16_value(uint16_t m, int e) { return ldexpf((float)(int16_t)m, e - 15); }
+static float bfp16_value(uint16_t m, int k) { return ldexpf((float)(int16_t)m, k); }
 
 void oracle_encode16(int fmt, const float* src, uint16_t* dst, int8_t* exp, size_t n, float scale) {
     if (fmt == 3) {
         for (size_t b = 0; b < n / 16; ++b) {
             float m = 0.0f;
             for (int k = 0; k < 16; ++k) m = fmaxf(m, fabsf(src[16 * b + k]));
-            const int e = bfp16_exponent(m);
-            exp[b] = (int8_t)e;
-            for (int k = 0; k < 16; ++k) dst[16 * b + k] = bfp16_mantissa(src[16 * b + k], e);
+            const int kb = bfp16_exponent(m);
+            exp[b] = (int8_t)kb;
+            for (int k = 0; k < 16; ++k) dst[16 * b + k] = bfp16_mantissa(src[16 * b + k], kb);
         }
         return;
     }

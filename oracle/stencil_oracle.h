@@ -104,10 +104,10 @@ double oracle_rel_l2_interior_f64f32(const float* a, const double* b, size_t nx,
 /* ---- 16-bit field storage (no reference counterpart: the reference only
  * has the compile-time FP64 switch).  Formats: 1 bfloat16 (round to nearest
  * even), 2 IEEE half of value*scale (scale a power of two), 3 block floating
- * point (int16 mantissa per value, one int8 exponent e per 16 x-consecutive
- * values, v = m * 2^(e-15), e = frexp exponent of the block's max |v|, 0 for
- * an all-zero block, clamped to [-112, 127] so that 2^(15-e) and 2^(e-15)*32767
- * are normal floats; |m| <= 32767). ---- */
+ * point (int16 mantissa m per value, one int8 block scale k per 16
+ * x-consecutive values, v = m * 2^k; k = e - 15 with e the frexp exponent of
+ * fl(max|v| * (1 + 2^-15)) clamped to [-112, 127] (e = 0 for an all-zero
+ * block), so 2^-k is a normal float and |m| <= 32767 without clamping). ---- */
 uint16_t oracle_f32_to_bf16(float f);
 float oracle_bf16_to_f32(uint16_t h);
 uint16_t oracle_f32_to_fp16(float f);

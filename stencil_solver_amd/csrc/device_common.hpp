@@ -55,10 +55,10 @@ __device__ __forceinline__ void store_stream_u2(uint2* p, uint2 v) {
 // ── 16-bit field storage (arithmetic stays FP32; decoding is exact) ─────────
 // kBf16:  bfloat16, round to nearest even on store.
 // kFp16s: IEEE half of value * scale, scale a power of two per field.
-// kBfp16: block floating point: int16 mantissas, one shared power-of-two
-//         exponent e (int8) per 16 x-consecutive values: v = m * 2^(e-15),
-//         e = frexp exponent of the block's max |v| clamped to [-112, 127]
-//         (0 for an all-zero block), |m| <= 32767.
+// kBfp16: block floating point: int16 mantissas m, one int8 block scale k per
+//         16 x-consecutive values: v = m * 2^k; k = e - 15 with e the frexp
+//         exponent of fl(max|v| * (1 + 2^-15)) clamped to [-112, 127] (0 for
+//         an all-zero block), so |m| <= 32767 and 2^-k is a normal float.
 // Stores use gfx950's packed converts (v_cvt_pk_bf16_f32, v_cvt_pk_f16_f32:
 // two values per instruction, round to nearest even); loads are one shift,
 // mask or SDWA convert per value.
@@ -113,32 +113,32 @@ __device__ __forceinline__ uint2 float4_to_fp16x4(float4 v, float scale) {
     return make_uint2(pack_fp16(v.x * scale, v.y * scale), pack_fp16(v.z * scale, v.w * scale));
 }
 
-__device__ __forceinline__ float4 bfp16x4_to_float4(uint2 u, int e) {
-    const int k = e - 15;
-    return make_float4(__builtin_ldexpf(static_cast<float>(static_cast<short>(u.x)), k),
-                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.x >> 16)), k),
-                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.y)), k),
-                       __builtin_ldexpf(static_cast<float>(static_cast<short>(u.y >> 16)), k));
+// The 4 mantissas as floats (exact); the block scale 2^k is applied by the
+// caller.
+__device__ __forceinline__ float4 bfp16x4_mantissas(uint2 u) {
+    return make_float4(static_cast<float>(static_cast<short>(u.x)), static_cast<float>(static_cast<short>(u.x >> 16)),
+                       static_cast<float>(static_cast<short>(u.y)), static_cast<float>(static_cast<short>(u.y >> 16)));
 }
-// Mantissas of a block with exponent e: rint(clamp(v * 2^(15-e), +-32767)).
-// Clamping v to +-32767 * 2^(e-15) first is the same clamp (both scalings are
-// exact), and adding 1.5 * 2^23 in the fma rounds v * 2^(15-e) to an integer
-// (ulp 1, ties to even) whose two's-complement low 16 bits are the mantissa.
-// With e in [-112, 127] both 2^(15-e) and the limit are normal floats.
-__device__ __forceinline__ uint2 float4_to_bfp16x4(float4 v, int e) {
-    const float s = __builtin_ldexpf(1.0f, 15 - e);
-    const float lim = __builtin_ldexpf(32767.0f, e - 15);
-    auto q = [s, lim](float x) {
-        return __float_as_uint(__builtin_fmaf(__builtin_amdgcn_fmed3f(x, -lim, lim), s, 12582912.0f));
-    };
+__device__ __forceinline__ float4 bfp16x4_to_float4(uint2 u, int k) {
+    const float4 m = bfp16x4_mantissas(u);
+    return make_float4(__builtin_ldexpf(m.x, k), __builtin_ldexpf(m.y, k), __builtin_ldexpf(m.z, k),
+                       __builtin_ldexpf(m.w, k));
+}
+// Mantissas of a block with scale k: rint(v * 2^-k), |v * 2^-k| < 32767.5 by
+// the choice of k.  Adding 1.5 * 2^23 in the fma rounds v * 2^-k (exact, a
+// power-of-two scaling) to an integer (ulp 1, ties to even) whose
+// two's-complement low 16 bits are the mantissa.
+__device__ __forceinline__ uint2 float4_to_bfp16x4(float4 v, int k) {
+    const float s = __uint_as_float(static_cast<unsigned>(127 - k) << 23);  // 2^-k
+    auto q = [s](float x) { return __float_as_uint(__builtin_fmaf(x, s, 12582912.0f)); };
     return make_uint2(__builtin_amdgcn_perm(q(v.y), q(v.x), 0x05040100u),
                       __builtin_amdgcn_perm(q(v.w), q(v.z), 0x05040100u));
 }
-// Shared exponent of a block whose max |v| is m >= 0: frexp exponent
-// (m < 2^e; v_frexp_exp gives 0 for m == 0) clamped to [-112, 127].
+// Block scale of a block whose max |v| is m >= 0 (v_frexp_exp gives 0 for 0).
 __device__ __forceinline__ int bfp16_exponent(float m) {
-    const int e = __builtin_amdgcn_frexp_expf(m);
-    return e < -112 ? -112 : (e > 127 ? 127 : e);
+    int e = __builtin_amdgcn_frexp_expf(m * (1.0f + 0x1p-15f));
+    e = e < -112 ? -112 : (e > 127 ? 127 : e);
+    return e - 15;
 }
 __device__ __forceinline__ float max_abs4(float4 v) {
     return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));

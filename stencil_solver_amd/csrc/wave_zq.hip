@@ -102,10 +102,10 @@ __device__ __forceinline__ float4 decode(Raw<FMT> u, int e, float inv) {
 }
 
 // Stores 4 values at byte offset boff of a plane.  Every lane of a wave
-// reaches this (kBfp16 shares one exponent across a quad of lanes); `vals`
-// says whether this lane writes its values, `blk` whether its quad's block
-// holds outputs (then lane 0 of the quad writes the exponent at byte eoff of
-// the plane's exponent row `er`).
+// reaches this (kBfp16 shares one block scale across a quad of lanes); `vals`
+// says whether this lane writes its values (v is zero where it does not),
+// `blk` whether its quad's block holds outputs (then lane 0 of the quad writes
+// the scale at byte eoff of the plane's scale row `er`).
 template <int FMT, bool NT, bool SCALED>
 __device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned eoff, float4 v,
                                        float scale, bool vals, bool blk) {
@@ -121,8 +121,8 @@ __device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned 
         uint2 u;
         int e = 0;
         if constexpr (FMT == dev::kBfp16) {
-            const float m = vals ? dev::max_abs4(v) : 0.f;
-            e = dev::bfp16_exponent(dev::quad_max(m));
+            // v is zero in lanes that store nothing, so they do not raise the max
+            e = dev::bfp16_exponent(dev::quad_max(dev::max_abs4(v)));
             u = dev::float4_to_bfp16x4(v, e);
         } else if constexpr (FMT == dev::kBf16) {
             u = dev::float4_to_bf16x4(v);
@@ -261,7 +261,9 @@ void zq_kernel(ZqArgs a) {
 
     // One plane of the star stencil for this thread's 4 points: LDS buffer b
     // holds plane zz (x and y neighbours), zq[k] is plane zz - 4 + k.
-    auto compute = [&](int b, const float4 (&zq)[9], float4 pv, float4 vv) -> float4 {
+    // dt2v multiplies vv in the fused forms (dt^2, or dt^2 * 2^k when vv holds
+    // the raw mantissas of a block-FP16 vel2 block).
+    auto compute = [&](int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v) -> float4 {
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ctr4 = zq[kR];
         const float4 xl = lds[b][ty + kR][tx];
@@ -304,7 +306,7 @@ void zq_kernel(ZqArgs a) {
                     res = acc;
                 } else {
                     const float base = __builtin_fmaf(2.f, p, -comp(pv, i));
-                    res = __builtin_fmaf(a.s.dt2 * comp(vv, i), acc, base);
+                    res = __builtin_fmaf(dt2v * comp(vv, i), acc, base);
                 }
             }
             dev::set_comp(out, i, res);
@@ -313,9 +315,9 @@ void zq_kernel(ZqArgs a) {
     };
     // Steps past z_end (the tail of the last group of 9 in a chunk whose length
     // is not a multiple of 9) run on clamped planes and store nothing.
-    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 pv, float4 vv) {
+    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v) {
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own_out) out = compute(b, zq, pv, vv);
+        if (own_out) out = compute(b, zq, pv, vv, dt2v);
         if (zz < z_end)
             store4<FMT, NT, kExact>(plane_rsrc(next_b, zz, plane_bytes),
                                     plane_rsrc(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
@@ -363,7 +365,7 @@ void zq_kernel(ZqArgs a) {
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv_cur, vv_cur);
+                finish(zz, b, zq, pv_cur, vv_cur, a.s.dt2);
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k) h_cur[k] = h_nxt[k];
                 pv_cur = pv_nxt;
@@ -375,14 +377,23 @@ void zq_kernel(ZqArgs a) {
         // words (the queue refill for plane zz+5, halo and streams for plane
         // zz+1) and decoded at the top of the step that uses it, so the raw
         // registers are free again before they are reloaded.
+        // Halo items exist for the first kHalo threads (whole waves but one),
+        // so the others skip their loads and decodes.
+        bool h_use[kHaloPer];
+#pragma unroll
+        for (int k = 0; k < kHaloPer; ++k) h_use[k] = tid + k * kThreads < kHalo;
         Raw<FMT> rq = raw_at(cur_b, z_begin + kR, own_ld);
         int rq_e = exp_at(a.cur_e, z_begin + kR, own_eld);
         Raw<FMT> rh[kHaloPer];
         int rh_e[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
-            rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
-            rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
+            rh[k] = Raw<FMT>{};
+            rh_e[k] = 0;
+            if (h_use[k]) {
+                rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
+                rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
+            }
         }
         Raw<FMT> rp = raw_stream(prev_b, z_begin), rv = raw_stream(vel_b, z_begin);
         int rp_e = exp_at(a.prev_e, z_begin, own_eld);
@@ -396,16 +407,26 @@ void zq_kernel(ZqArgs a) {
                 q[(j + 8) % 9] = decode<FMT, kExact>(rq, rq_e, p_inv);
                 float4 h[kHaloPer];
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) h[k] = decode<FMT, kExact>(rh[k], rh_e[k], p_inv);
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (h_use[k]) h[k] = decode<FMT, kExact>(rh[k], rh_e[k], p_inv);
                 const float4 pv = decode<FMT, kExact>(rp, rp_e, p_inv);
-                const float4 vv = decode<FMT, kExact>(rv, rv_e, v_inv);
+                float4 vv;
+                float dt2v = a.s.dt2;
+                if constexpr (kBlk && !kExact) {
+                    // fused form: fold vel2's block scale into dt^2
+                    vv = dev::bfp16x4_mantissas(rv);
+                    dt2v = __builtin_ldexpf(a.s.dt2, rv_e);
+                } else {
+                    vv = decode<FMT, kExact>(rv, rv_e, v_inv);
+                }
                 rq = raw_at(cur_b, z5, own_ld);
                 rq_e = exp_at(a.cur_e, z5, own_eld);
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) {
-                    rh[k] = raw_at(cur_b, z1, h_ld[k]);
-                    rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
-                }
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (h_use[k]) {
+                        rh[k] = raw_at(cur_b, z1, h_ld[k]);
+                        rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
+                    }
                 rp = raw_stream(prev_b, z1);
                 rv = raw_stream(vel_b, z1);
                 rp_e = exp_at(a.prev_e, z1, own_eld);
@@ -414,12 +435,12 @@ void zq_kernel(ZqArgs a) {
                 lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
-                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h[k];
+                    if (h_use[k]) lds[b][h_row[k]][h_col[k]] = h[k];
                 __syncthreads();
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv, vv);
+                finish(zz, b, zq, pv, vv, dt2v);
             }
         }
     }

@@ -62,36 +62,44 @@ def test_bf16_round_to_nearest_even(oracle):
 
 
 def test_bfp16_block_format(oracle):
-    """Shared exponent = frexp exponent of the block max, clamped to
-    [-112, 127]; each mantissa is the nearest integer (ties to even) of
-    v*2^(15-e), |m| <= 32767; decode exact."""
+    """Block scale k = e - 15, e the frexp exponent of fl32(max * (1 + 2^-15))
+    clamped to [-112, 127] (0 for an all-zero block); each mantissa is the
+    nearest integer (ties to even) of v * 2^-k and never needs clamping;
+    decode exact."""
     rng = np.random.default_rng(3)
     blocks = [rng.standard_normal(16).astype(np.float32) * s
               for s in (1.0, 1e-3, 1e4, 1e-30, 1e-34, 1e-36, 1e-40)]
-    # a block max just below 2^e rounds up to 2^15: clamped to 32767
-    blocks.append(np.array([np.nextafter(np.float32(1.0), np.float32(0.0))] + [0.25] * 15, np.float32))
     blocks.append(np.zeros(16, np.float32))
     blocks.append(np.array([0.5] + [0.0] * 15, np.float32))
     blocks.append(np.array([-1.0] * 16, np.float32))
+    # maxima just below a power of two: rounding v * 2^(15-e) would give 2^15,
+    # so the exponent moves up one
+    blocks.append(np.array([np.nextafter(np.float32(1.0), np.float32(0.0))] + [0.25] * 15, np.float32))
+    blocks.append(np.array([-np.float32(1.0 - 2.0 ** -17)] + [0.1] * 15, np.float32))
+    # and just far enough below that it stays (v * 2^15 = 32766.75 -> |m| = 32767)
+    blocks.append(np.array([np.float32(1.0 - 5 * 2.0 ** -17)] + [0.0] * 15, np.float32))
     x = np.concatenate(blocks)
     d, e = oracle.encode16("bfp16", x)
     back = oracle.decode16("bfp16", d, e)
     for b in range(len(blocks)):
         blk = x[16 * b:16 * b + 16].astype(np.float64)
-        m = np.max(np.abs(blk))
-        want_e = 0 if m == 0 else min(127, max(-112, int(np.frexp(m)[1])))
-        assert int(e[b]) == want_e
+        m = np.float32(np.max(np.abs(blk)))
+        if m == 0:
+            want_k = -15
+        else:
+            m2 = np.float32(m * np.float32(1.0 + 2.0 ** -15))  # one float32 rounding
+            want_k = min(127, max(-112, int(np.frexp(np.float64(m2))[1]))) - 15
+        assert int(e[b]) == want_k, b
         for k in range(16):
-            t = Fraction(float(blk[k])) * Fraction(2) ** (15 - want_e)
+            t = Fraction(float(blk[k])) * Fraction(2) ** (-want_k)
             q = round(t)  # Python rounds half to even
-            q = max(-32767, min(32767, q))
+            assert abs(q) <= 32767
             assert int(d[16 * b + k].astype(np.int16)) == q
-            assert Fraction(float(back[16 * b + k])) == Fraction(q) * Fraction(2) ** (want_e - 15)
-        if m > 0:  # half a mantissa step, a whole one where 2^15 was clamped to 32767
-            err = np.abs(back[16 * b:16 * b + 16] - blk)
-            clamped = np.abs(blk) * 2.0 ** (15 - want_e) > 32767
-            assert np.all(err[~clamped] <= 2.0 ** (want_e - 16) * (1 + 1e-12))
-            assert np.all(err[clamped] <= 2.0 ** (want_e - 15))
+            assert Fraction(float(back[16 * b + k])) == Fraction(q) * Fraction(2) ** want_k
+        if m > 0:  # half a mantissa step
+            assert np.max(np.abs(back[16 * b:16 * b + 16] - blk)) <= 2.0 ** (want_k - 1) * (1 + 1e-12)
+    assert int(d[16 * (len(blocks) - 1)].astype(np.int16)) == 32767
+    assert int(e[len(blocks) - 3]) == 1 - 15  # 1 - 2^-24 moved up to e = 1
 
 
 def test_fp16_scaled_roundtrip(oracle):
