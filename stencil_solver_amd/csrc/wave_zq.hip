@@ -378,7 +378,9 @@ void zq_kernel(ZqArgs a) {
         // zz+1) and decoded at the top of the step that uses it, so the raw
         // registers are free again before they are reloaded.
         // Halo items exist for the first kHalo threads (whole waves but one),
-        // so the others skip their loads and decodes.
+        // so the others skip the decodes.  The loads stay unconditional: under
+        // a divergent branch the compiler stops treating the plane descriptors
+        // as uniform and wraps every buffer access in a readfirstlane loop.
         bool h_use[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) h_use[k] = tid + k * kThreads < kHalo;
@@ -388,12 +390,8 @@ void zq_kernel(ZqArgs a) {
         int rh_e[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
-            rh[k] = Raw<FMT>{};
-            rh_e[k] = 0;
-            if (h_use[k]) {
-                rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
-                rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
-            }
+            rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
+            rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
         }
         Raw<FMT> rp = raw_stream(prev_b, z_begin), rv = raw_stream(vel_b, z_begin);
         int rp_e = exp_at(a.prev_e, z_begin, own_eld);
@@ -422,11 +420,10 @@ void zq_kernel(ZqArgs a) {
                 rq = raw_at(cur_b, z5, own_ld);
                 rq_e = exp_at(a.cur_e, z5, own_eld);
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k)
-                    if (h_use[k]) {
-                        rh[k] = raw_at(cur_b, z1, h_ld[k]);
-                        rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
-                    }
+                for (int k = 0; k < kHaloPer; ++k) {
+                    rh[k] = raw_at(cur_b, z1, h_ld[k]);
+                    rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
+                }
                 rp = raw_stream(prev_b, z1);
                 rv = raw_stream(vel_b, z1);
                 rp_e = exp_at(a.prev_e, z1, own_eld);
