@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage / FP64 study")
     ap.add_argument("--precision-n", type=int, nargs="*", default=None,
                     help="grid sizes of the 16-bit storage study (default: --n)")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the adjoint FWI gradient timing")
+    ap.add_argument("--fwi-n", type=int, default=256, help="FWI grid cells per axis")
+    ap.add_argument("--fwi-steps", type=int, default=2000, help="FWI time steps")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 2 STREAM copies of one field (known bytes) for counter scaling")
     ap.add_argument("--tune-csv", default=None, help="write the autotuner sweep to this CSV")
@@ -277,6 +280,9 @@ def main():
                 "best_16bit_gpts": ok[best]["gpts"] if best else None,
                 "speedup_vs_fp32": round(ok[best]["gpts"] / gpts, 3) if best else None}
 
+    if not args.no_fwi:
+        result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev)
+
     if args.calibrate:
         # Known-size copy: n^3 floats read + n^3 floats written per launch, the
         # reference the FETCH_SIZE / WRITE_SIZE counters are scaled against.
@@ -391,6 +397,56 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     print(f"[bench] precision {n}^3 x {steps}: " +
           ", ".join(f"{k}: {v.get('gpts', '-')} Gpts/s rel {v.get('rel_l2_vs_fp64', 0):.2e}" for k, v in out.items()
                     if k != "fp64_ref"), file=sys.stderr, flush=True)
+    return out
+
+
+def fwi_study(ss, n, steps, dev):
+    """Adjoint FWI velocity gradient: n^3 FP32, h = 10 m, dt = 8e-4 s.  True
+    model: seeded smooth noise (sigma 10 cells) mapped to [1800, 4000] m/s plus
+    a +10 % sphere of radius n/12.8 cells at the centre; starting model without
+    it.  12 Hz Ricker at (z, y, x) = (12, n/2, n/2); receivers: the whole plane
+    z = 8.  grad time = checkpointed forward (misfit) + adjoint sweep."""
+    import math
+
+    import torch
+    from stencil_solver_amd import synth
+
+    shape = (n, n, n)
+    h, dt = 10.0, 8e-4
+    grid = ss.GridDims.cube(n, h, dt)
+    v0 = synth.smooth_velocity(shape, 10.0, 1800.0, 4000.0, seed=5, device=dev)
+    sphere = synth.sphere_mask(shape, (n // 2, n // 2, n // 2), n / 12.8, device=dev)
+    v_true = torch.where(sphere, v0 * 1.1, v0)
+    m0, m_true = (v0 * v0).contiguous(), (v_true * v_true).contiguous()
+    w = ss.ricker(steps, dt, 12.0)
+    pr = ss.FwiProblem(grid, steps, (12, n // 2, n // 2), 8, w, dev, torch.float32)
+    d_obs = pr.traces(m_true)
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    pr.misfit(m0, d_obs)  # warm-up (first-launch costs)
+    pr.gradient(m0)
+    torch.cuda.synchronize()
+    e[0].record()
+    J = pr.misfit(m0, d_obs)
+    e[1].record()
+    g = pr.gradient(m0)
+    e[2].record()
+    torch.cuda.synchronize()
+    fwd_ms, bwd_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+    interior = (n - 8) ** 3
+    # forward 16 B/pt, recompute 16 + 4 (L p), adjoint 28 (lambda1, lambda2,
+    # m, L p, grad in; lambda, grad out) per interior point and step
+    alg = 64.0 * interior * steps
+    grad_ms = fwd_ms + bwd_ms
+    out = {"grid": n, "steps": steps, "ckpt_every": pr.ckpt_every, "dtype": "f32",
+           "grad_ms": round(grad_ms, 2), "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+           "misfit": float(J), "grad_l2": float(g.double().norm()),
+           "workspace_gb": round(pr.workspace_bytes / 1e9, 3),
+           "algorithmic_bytes": alg, "achieved_gbs": round(alg / (grad_ms * 1e-3) / 1e9, 1),
+           "grad_floor_ms_at_8tbs": round(alg / 8e12 * 1e3, 1)}
+    print(f"[bench] fwi {n}^3 x {steps}: grad {grad_ms:.1f} ms (forward {fwd_ms:.1f}, backward {bwd_ms:.1f}), "
+          f"{out['achieved_gbs']} GB/s algorithmic", file=sys.stderr, flush=True)
+    del pr, d_obs, g, m0, m_true, v0, v_true
+    torch.cuda.empty_cache()
     return out
 
 

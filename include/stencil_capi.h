@@ -232,6 +232,67 @@ int stencil_host_layered_noise_vel2_f32(float* host, uint64_t nx, uint64_t ny, u
 int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t nz,
                                 uint64_t side, float hot, uint64_t seed);
 
+/* ---- Adjoint-state FWI gradient (no reference counterpart: the reference
+ * models forward only).  Scheme, with m = vel2, L the stencil Laplacian and a
+ * point source of amplitude wavelet[n] added to p^{n+1}:
+ *   p^{n+1} = 2p^n - p^{n-1} + dt^2 m L p^n + wavelet[n] e_src,  p^0 = p^-1 = 0
+ *   J = 1/2 sum_{k=1..N} |p^k(plane rec_z) - d_obs^k|^2
+ * The gradient is the exact discrete adjoint: dJ/dm = sum_k dt^2 (L p^k)
+ * lambda^{k+1}, lambda^k = 2lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1})
+ * + residual^k on plane rec_z.  The forward pass keeps the level pair at
+ * every ckpt_every-th step; the backward pass recomputes each segment from
+ * it.  All device memory is the caller's workspace.  Arrays: vel2 and grad
+ * nx*ny*nz, wavelet n_steps (device), traces / d_obs n_steps*ny*nx with trace
+ * n holding p^{n+1} on plane rec_z. ---- */
+typedef struct stencil_fwi_config {
+    uint64_t n_steps;
+    uint64_t src_x, src_y, src_z; /* source cell (interior) */
+    uint64_t rec_z;               /* receiver plane (interior z), all nx*ny cells */
+    uint64_t ckpt_every;          /* segment length; 0 = ceil(sqrt(n_steps)) */
+} stencil_fwi_config;
+/* Workspace bytes (256-byte aligned base) for this grid, config and value
+ * size (4 or 8); 0 if the arguments are invalid. */
+uint64_t stencil_fwi_workspace_bytes(const stencil_grid* grid, const stencil_fwi_config* cfg,
+                                     int bytes_per_value);
+/* Forward modelling with checkpoints.  traces_out (nullable) receives the
+ * synthetic traces; with d_obs (nullable) the residual is kept in the
+ * workspace and J is written to misfit_out (device double, nullable). */
+int stencil_fwi_forward_f32(const float* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                            const float* wavelet, const float* d_obs, float* traces_out, double* misfit_out,
+                            void* work, void* hip_stream);
+int stencil_fwi_forward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                            const double* wavelet, const double* d_obs, double* traces_out, double* misfit_out,
+                            void* work, void* hip_stream);
+/* Adjoint sweep after a forward call with d_obs on the same workspace: grad
+ * (nx*ny*nz) = dJ/dvel2; src_trace (nullable, n_steps) = lambda^{n+1} at the
+ * source cell, the adjoint of the source injection (dot-product test). */
+int stencil_fwi_backward_f32(const float* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                             const float* wavelet, float* grad, float* src_trace, void* work,
+                             void* hip_stream);
+int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                             const double* wavelet, double* grad, double* src_trace, void* work,
+                             void* hip_stream);
+/* The single steps: the acoustic step (reference association) plus
+ * wavelet[n] at linear cell src_index (< 0: none), writing L p_cur into
+ * lap_out (nullable); and the adjoint step, lam_out = 2 lam1 - lam2 +
+ * dt^2 L(vel2 lam1) (+ res_plane on plane rec_z when non-null), grad +=
+ * dt^2 lap lam1 when lap is non-null, *src_value = lam_out[src_index] when
+ * src_value is non-null. */
+int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const float* vel2, float* p_next,
+                             float* lap_out, const stencil_grid* grid, int64_t src_index,
+                             const float* wavelet, int64_t n, void* hip_stream);
+int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const double* vel2,
+                             double* p_next, double* lap_out, const stencil_grid* grid, int64_t src_index,
+                             const double* wavelet, int64_t n, void* hip_stream);
+int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap,
+                             float* grad, float* lam_out, const float* res_plane, int64_t rec_z,
+                             const stencil_grid* grid, int64_t src_index, float* src_value,
+                             void* hip_stream);
+int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
+                             double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
+                             const stencil_grid* grid, int64_t src_index, double* src_value,
+                             void* hip_stream);
+
 /* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)
  * for callers that cannot use the C++ API. ---- */
 typedef struct stencil_config {

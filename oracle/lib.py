@@ -75,6 +75,12 @@ class Oracle:
         L.oracle_encode16.argtypes = [C.c_int, _P, _P, _P, _SZ, _F]
         L.oracle_decode16.argtypes = [C.c_int, _P, _P, _P, _SZ, _F]
         L.oracle_wave_step16.argtypes = [C.c_int] + [_P] * 8 + [_SZ] * 3 + [_F] * 6
+        for suf, T in (("f32", _F), ("f64", _D)):
+            getattr(L, f"oracle_fwi_forward_{suf}").argtypes = [_P] + [_SZ] * 3 + [T] * 4 + [C.c_int, _SZ, _SZ,
+                                                                                              _P, _P]
+            g = getattr(L, f"oracle_fwi_gradient_{suf}")
+            g.argtypes = [_P] + [_SZ] * 3 + [T] * 4 + [C.c_int, _SZ, _SZ, _P, _P, _P, _P]
+            g.restype = _D
 
     # -- coefficients ------------------------------------------------------
     def fd_coeffs(self, dtype=np.float32) -> np.ndarray:
@@ -188,6 +194,31 @@ class Oracle:
         self.lib.oracle_wave_step16(f, _ptr(cur[0]), ex(cur), _ptr(prev[0]), ex(prev), _ptr(vel2[0]), ex(vel2),
                                     _ptr(nxt[0]), ex(nxt), nx, ny, nz, dx, dy, dz, dt, p_scale, v_scale)
         return nxt
+
+    # -- adjoint-state FWI ------------------------------------------------
+    def fwi_forward(self, m, h, dt, n_steps, src, rec_z, wavelet):
+        """Traces (n_steps, ny, nx): trace n = p^{n+1} on plane rec_z.  src is (z, y, x)."""
+        nz, ny, nx = m.shape
+        T = m.dtype
+        out = np.zeros((n_steps, ny, nx), dtype=T)
+        w = np.ascontiguousarray(wavelet, dtype=T)
+        f = self.lib.oracle_fwi_forward_f32 if T == np.float32 else self.lib.oracle_fwi_forward_f64
+        si = (src[0] * ny + src[1]) * nx + src[2]
+        f(_ptr(m), nx, ny, nz, h, h, h, dt, n_steps, si, rec_z, _ptr(w), _ptr(out))
+        return out
+
+    def fwi_gradient(self, m, h, dt, n_steps, src, rec_z, wavelet, d_obs):
+        """(J, dJ/dm, lambda^{n+1} at src) of J = 1/2 |traces - d_obs|^2."""
+        nz, ny, nx = m.shape
+        T = m.dtype
+        grad = np.zeros_like(m)
+        st = np.zeros(n_steps, dtype=T)
+        w = np.ascontiguousarray(wavelet, dtype=T)
+        d = np.ascontiguousarray(d_obs, dtype=T)
+        f = self.lib.oracle_fwi_gradient_f32 if T == np.float32 else self.lib.oracle_fwi_gradient_f64
+        si = (src[0] * ny + src[1]) * nx + src[2]
+        J = f(_ptr(m), nx, ny, nz, h, h, h, dt, n_steps, si, rec_z, _ptr(w), _ptr(d), _ptr(grad), _ptr(st))
+        return J, grad, st
 
     # -- solver runs -------------------------------------------------------
     def wave_run(self, p0, vel2, dx, dt, n_steps, dy=None, dz=None):

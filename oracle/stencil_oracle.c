@@ -457,3 +457,134 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
     free(fv);
     free(fn);
 }
+
+/* ---- Adjoint-state FWI (no reference counterpart).  Restates the scheme of
+ * the library's FWI path with every time level kept (no checkpointing), in the
+ * same association as its kernels:
+ *   forward  p^{n+1} = 2p^n - p^{n-1} + (dt2*m)*L p^n  (+ wavelet[n] at src)
+ *   L p      = (lx + ly) + lz,  la = c0*p*inv + sum_r c_r*(p[+r]+p[-r])*inv
+ *   adjoint  lambda^k = 2 lambda^{k+1} - lambda^{k+2} + dt2*L(m lambda^{k+1})
+ *                       (+ res^k on plane rec_z)
+ *   gradient g += (dt2 * L p^k) * lambda^{k+1}, k = N .. 1
+ * Pinned by tests/test_oracle_fwi.py: central finite differences of the misfit
+ * (FP64), the adjoint dot-product identity, and the forward step against the
+ * pinned oracle_wave_step. ---- */
+#define ORACLE_FWI(T, SUF, COEFFS)                                                                       \
+    static void fwi_lap_##SUF(const T* p, const T* m, T* out, int NX, int NY, int NZ, T ix2, T iy2,     \
+                              T iz2) {                                                                  \
+        T c[5];                                                                                         \
+        COEFFS(c);                                                                                      \
+        const ptrdiff_t sy = NX, sz = (ptrdiff_t)NX * NY;                                               \
+        for (int iz = R; iz < NZ - R; ++iz)                                                             \
+            for (int iy = R; iy < NY - R; ++iy)                                                         \
+                for (int ix = R; ix < NX - R; ++ix) {                                                   \
+                    const ptrdiff_t q = iz * sz + iy * sy + ix;                                         \
+                    const T ctr = m ? m[q] * p[q] : p[q];                                               \
+                    T lx = c[0] * ctr * ix2, ly = c[0] * ctr * iy2, lz = c[0] * ctr * iz2;              \
+                    for (int r = 1; r <= R; ++r) {                                                      \
+                        const ptrdiff_t o[3] = {r, r * sy, r * sz};                                     \
+                        T a[3], b[3];                                                                   \
+                        for (int k = 0; k < 3; ++k) {                                                   \
+                            a[k] = m ? m[q + o[k]] * p[q + o[k]] : p[q + o[k]];                         \
+                            b[k] = m ? m[q - o[k]] * p[q - o[k]] : p[q - o[k]];                         \
+                        }                                                                               \
+                        lx += c[r] * (a[0] + b[0]) * ix2;                                               \
+                        ly += c[r] * (a[1] + b[1]) * iy2;                                               \
+                        lz += c[r] * (a[2] + b[2]) * iz2;                                               \
+                    }                                                                                   \
+                    out[q] = lx + ly + lz;                                                              \
+                }                                                                                       \
+    }                                                                                                   \
+    /* levels: (n_steps+1) fields p^0..p^N (p^0 = 0). */                                               \
+    static void fwi_levels_##SUF(const T* m, int NX, int NY, int NZ, T dx, T dy, T dz, T dt, int N,      \
+                                 size_t src, const T* wavelet, T* levels, T* laps) {                    \
+        const size_t n = (size_t)NX * NY * NZ;                                                          \
+        const T ix2 = (T)1 / (dx * dx), iy2 = (T)1 / (dy * dy), iz2 = (T)1 / (dz * dz), dt2 = dt * dt; \
+        T* zero = (T*)calloc(n, sizeof(T));                                                             \
+        memset(levels, 0, n * sizeof(T));                                                               \
+        for (int s = 0; s < N; ++s) {                                                                   \
+            const T* p = levels + (size_t)s * n;                                                        \
+            const T* pp = s > 0 ? levels + (size_t)(s - 1) * n : zero;                                  \
+            T* pn = levels + (size_t)(s + 1) * n;                                                       \
+            T* lap = laps + (size_t)s * n;                                                              \
+            memset(pn, 0, n * sizeof(T));                                                               \
+            fwi_lap_##SUF(p, NULL, lap, NX, NY, NZ, ix2, iy2, iz2);                                     \
+            for (int iz = R; iz < NZ - R; ++iz)                                                         \
+                for (int iy = R; iy < NY - R; ++iy)                                                     \
+                    for (int ix = R; ix < NX - R; ++ix) {                                               \
+                        const size_t q = ((size_t)iz * NY + iy) * NX + ix;                              \
+                        T v = (T)2 * p[q] - pp[q] + dt2 * m[q] * lap[q];                                \
+                        if (q == src) v += wavelet[s];                                                  \
+                        pn[q] = v;                                                                      \
+                    }                                                                                   \
+        }                                                                                               \
+        free(zero);                                                                                     \
+    }                                                                                                   \
+    void oracle_fwi_forward_##SUF(const T* m, size_t nx, size_t ny, size_t nz, T dx, T dy, T dz, T dt,  \
+                                  int n_steps, size_t src, size_t rec_z, const T* wavelet, T* traces) { \
+        const size_t n = nx * ny * nz, plane = nx * ny;                                                 \
+        T* levels = (T*)malloc((size_t)(n_steps + 1) * n * sizeof(T));                                  \
+        T* laps = (T*)malloc((size_t)(n_steps + 1) * n * sizeof(T));                                    \
+        fwi_levels_##SUF(m, (int)nx, (int)ny, (int)nz, dx, dy, dz, dt, n_steps, src, wavelet, levels,   \
+                         laps);                                                                         \
+        for (int k = 1; k <= n_steps; ++k)                                                              \
+            memcpy(traces + (size_t)(k - 1) * plane, levels + (size_t)k * n + rec_z * plane,            \
+                   plane * sizeof(T));                                                                  \
+        free(levels);                                                                                   \
+        free(laps);                                                                                     \
+    }                                                                                                   \
+    double oracle_fwi_gradient_##SUF(const T* m, size_t nx, size_t ny, size_t nz, T dx, T dy, T dz,     \
+                                     T dt, int n_steps, size_t src, size_t rec_z, const T* wavelet,     \
+                                     const T* d_obs, T* grad, T* src_trace) {                           \
+        const int NX = (int)nx, NY = (int)ny, NZ = (int)nz, N = n_steps;                                \
+        const size_t n = nx * ny * nz, plane = nx * ny;                                                 \
+        const T ix2 = (T)1 / (dx * dx), iy2 = (T)1 / (dy * dy), iz2 = (T)1 / (dz * dz), dt2 = dt * dt; \
+        T* levels = (T*)malloc((size_t)(N + 1) * n * sizeof(T));                                        \
+        T* laps = (T*)malloc((size_t)(N + 1) * n * sizeof(T));                                          \
+        fwi_levels_##SUF(m, NX, NY, NZ, dx, dy, dz, dt, N, src, wavelet, levels, laps);                 \
+        /* L p^N (the forward loop stops at p^N) */                                                     \
+        fwi_lap_##SUF(levels + (size_t)N * n, NULL, laps + (size_t)N * n, NX, NY, NZ, ix2, iy2, iz2);   \
+        T* res = (T*)malloc((size_t)N * plane * sizeof(T));                                             \
+        double J = 0.0;                                                                                 \
+        for (int k = 1; k <= N; ++k)                                                                    \
+            for (size_t i = 0; i < plane; ++i) {                                                        \
+                const T r = levels[(size_t)k * n + rec_z * plane + i] - d_obs[(size_t)(k - 1) * plane + i]; \
+                res[(size_t)(k - 1) * plane + i] = r;                                                   \
+                J += (double)r * (double)r;                                                             \
+            }                                                                                           \
+        J *= 0.5;                                                                                       \
+        T* l1 = (T*)calloc(n, sizeof(T));                                                               \
+        T* l2 = (T*)calloc(n, sizeof(T));                                                               \
+        T* lo = (T*)calloc(n, sizeof(T));                                                               \
+        T* lq = (T*)calloc(n, sizeof(T));                                                               \
+        memset(grad, 0, n * sizeof(T));                                                                 \
+        for (int k = N; k >= 1; --k) {                                                                  \
+            fwi_lap_##SUF(l1, m, lq, NX, NY, NZ, ix2, iy2, iz2);                                        \
+            const T* lp = laps + (size_t)k * n;                                                         \
+            for (int iz = R; iz < NZ - R; ++iz)                                                         \
+                for (int iy = R; iy < NY - R; ++iy)                                                     \
+                    for (int ix = R; ix < NX - R; ++ix) {                                               \
+                        const size_t q = ((size_t)iz * NY + iy) * NX + ix;                              \
+                        T v = (T)2 * l1[q] - l2[q] + dt2 * lq[q];                                       \
+                        if ((size_t)iz == rec_z) v += res[(size_t)(k - 1) * plane + (size_t)iy * nx + ix]; \
+                        lo[q] = v;                                                                      \
+                        grad[q] += dt2 * lp[q] * l1[q];                                                 \
+                    }                                                                                   \
+            if (src_trace) src_trace[k - 1] = lo[src];                                                  \
+            T* t = l2;                                                                                  \
+            l2 = l1;                                                                                    \
+            l1 = lo;                                                                                    \
+            lo = t;                                                                                     \
+        }                                                                                               \
+        free(levels);                                                                                   \
+        free(laps);                                                                                     \
+        free(res);                                                                                      \
+        free(l1);                                                                                       \
+        free(l2);                                                                                       \
+        free(lo);                                                                                       \
+        free(lq);                                                                                       \
+        return J;                                                                                       \
+    }
+
+ORACLE_FWI(float, f32, oracle_fd_coeffs_f32)
+ORACLE_FWI(double, f64, oracle_fd_coeffs_f64)

@@ -124,6 +124,28 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
                         uint16_t* next, int8_t* next_e, size_t nx, size_t ny, size_t nz, float dx,
                         float dy, float dz, float dt, float p_scale, float v_scale);
 
+/* ---- Adjoint-state FWI (no reference counterpart): forward modelling with a
+ * point source (wavelet[n] added to p^{n+1} at linear cell src) and the
+ * receiver plane rec_z recorded as traces[n] = p^{n+1}(rec_z, :, :); the
+ * misfit J = 1/2 sum |traces - d_obs|^2 (returned, FP64 accumulation) and its
+ * exact discrete gradient dJ/dm (m = vel2) by the adjoint recursion with
+ * every level kept.  src_trace (nullable) = lambda^{n+1} at src.  Tiny
+ * grids only: (n_steps+1) fields of levels and Laplacians are held. ---- */
+void oracle_fwi_forward_f32(const float* m, size_t nx, size_t ny, size_t nz, float dx, float dy,
+                            float dz, float dt, int n_steps, size_t src, size_t rec_z,
+                            const float* wavelet, float* traces);
+void oracle_fwi_forward_f64(const double* m, size_t nx, size_t ny, size_t nz, double dx, double dy,
+                            double dz, double dt, int n_steps, size_t src, size_t rec_z,
+                            const double* wavelet, double* traces);
+double oracle_fwi_gradient_f32(const float* m, size_t nx, size_t ny, size_t nz, float dx, float dy,
+                               float dz, float dt, int n_steps, size_t src, size_t rec_z,
+                               const float* wavelet, const float* d_obs, float* grad,
+                               float* src_trace);
+double oracle_fwi_gradient_f64(const double* m, size_t nx, size_t ny, size_t nz, double dx,
+                               double dy, double dz, double dt, int n_steps, size_t src,
+                               size_t rec_z, const double* wavelet, const double* d_obs,
+                               double* grad, double* src_trace);
+
 #ifdef __cplusplus
 }
 #endif

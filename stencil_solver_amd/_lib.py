@@ -51,6 +51,11 @@ class stencil_field16(C.Structure):
     _fields_ = [("data", C.c_void_p), ("exp", C.c_void_p)]
 
 
+class stencil_fwi_config(C.Structure):
+    _fields_ = [("n_steps", C.c_uint64), ("src_x", C.c_uint64), ("src_y", C.c_uint64), ("src_z", C.c_uint64),
+                ("rec_z", C.c_uint64), ("ckpt_every", C.c_uint64)]
+
+
 class stencil_perf(C.Structure):
     _fields_ = [("time_ms", C.c_double), ("bandwidth_gbs", C.c_double), ("gflops", C.c_double),
                 ("model_bandwidth_gbs", C.c_double), ("grid_size", C.c_uint64),
@@ -102,6 +107,15 @@ SIGNATURES = {
                                                  C.c_float, C.c_float, C.c_float, C.c_uint64]),
     "stencil_host_noise_cube_f32": (_I, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                          C.c_float, C.c_uint64]),
+    "stencil_fwi_workspace_bytes": (C.c_uint64, [_G, C.POINTER(stencil_fwi_config), _I]),
+    "stencil_fwi_forward_f32": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P, _P]),
+    "stencil_fwi_forward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P, _P]),
+    "stencil_fwi_backward_f32": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
+    "stencil_fwi_backward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
+    "stencil_forward_step_f32": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _P]),
+    "stencil_forward_step_f64": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _P]),
+    "stencil_adjoint_step_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _P]),
+    "stencil_adjoint_step_f64": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _P]),
     "stencil_config_default": (None, [C.POINTER(stencil_config)]),
     "stencil_solver_create": (_I, [C.POINTER(stencil_config), C.POINTER(_P)]),
     "stencil_solver_step": (_I, [_P]),

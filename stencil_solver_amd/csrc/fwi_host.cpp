@@ -1,0 +1,339 @@
+// Adjoint-state FWI gradient with sqrt(n) checkpointing (no reference
+// counterpart).  The discrete scheme and its adjoint are stated in fwi.hip.
+//
+// Forward pass: n_steps acoustic steps from p^0 = p^-1 = 0 with a point source,
+// the receiver plane of every level p^1..p^N copied into the trace array, and
+// the level pair (p^{sC}, p^{sC+1}) kept for every segment s (C = ckpt_every,
+// default ceil(sqrt(N))).  With observed data the traces become the residual
+// R p^k - d^k in place and J = 1/2 |residual|^2 (FP64 reduction).
+// Backward pass, segments last to first: the forward steps of segment s are
+// re-run from its checkpoint, writing L p^k for k = sC+1 .. min((s+1)C, N-1);
+// then the adjoint steps k = min((s+1)C, N) .. sC+1 consume them:
+//   lambda^k = 2 lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1}) + res^k
+//   grad    += dt^2 (L p^k) lambda^{k+1}
+// Device memory: all of it is the caller's workspace (stencil_fwi_workspace_bytes):
+// 3 wavefield levels, 3 adjoint levels, 2 * ceil(N/C) checkpoint fields, C
+// Laplacian fields, N receiver planes and the reduction scratch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "launchers.hpp"
+#include "stencil_capi.h"
+
+namespace stencil::hipk {
+
+hipError_t fwi_fwd_step_f32(const float*, const float*, const float*, float*, float*, const Dims&,
+                            const Scalars<float>&, long long, const float*, int, hipStream_t);
+hipError_t fwi_fwd_step_f64(const double*, const double*, const double*, double*, double*, const Dims&,
+                            const Scalars<double>&, long long, const double*, int, hipStream_t);
+hipError_t fwi_adj_step_f32(const float*, const float*, const float*, const float*, float*, float*, const float*,
+                            int, const Dims&, const Scalars<float>&, long long, float*, hipStream_t);
+hipError_t fwi_adj_step_f64(const double*, const double*, const double*, const double*, double*, double*,
+                            const double*, int, const Dims&, const Scalars<double>&, long long, double*,
+                            hipStream_t);
+hipError_t fwi_residual_f32(const float*, const float*, float*, size_t, double*, double*, hipStream_t);
+hipError_t fwi_residual_f64(const double*, const double*, double*, size_t, double*, double*, hipStream_t);
+hipError_t fill_f32(float*, size_t, float, hipStream_t);
+hipError_t fill_f64(double*, size_t, double, hipStream_t);
+
+}  // namespace stencil::hipk
+
+using namespace stencil::hipk;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr size_t kPartials = 1024;  // residual reduction blocks
+
+size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+struct Plan {
+    Dims d;
+    size_t cells = 0, plane = 0;
+    int64_t n = 0, C = 0, S = 0;
+    long long src = 0;
+    int rec_z = 0;
+    // workspace offsets (bytes)
+    size_t ring = 0, lam = 0, ckpt = 0, lap = 0, traces = 0, partial = 0, misfit = 0, total = 0;
+};
+
+bool make_plan(const stencil_grid* g, const stencil_fwi_config* c, size_t esz, Plan* p) {
+    if (!g || !c || g->nx <= 8 || g->ny <= 8 || g->nz <= 8 || c->n_steps == 0) return false;
+    if (g->nx > (1u << 20) || g->ny > (1u << 20) || g->nz > (1u << 20)) return false;
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !(g->dt > 0)) return false;
+    if (c->n_steps > (1u << 30)) return false;
+    auto inside = [](uint64_t v, uint64_t n) { return v >= 4 && v + 4 < n; };
+    if (!inside(c->src_x, g->nx) || !inside(c->src_y, g->ny) || !inside(c->src_z, g->nz)) return false;
+    if (!inside(c->rec_z, g->nz)) return false;
+    p->d.nx = int64_t(g->nx);
+    p->d.ny = int64_t(g->ny);
+    p->d.nz = int64_t(g->nz);
+    p->plane = size_t(g->nx) * size_t(g->ny);
+    p->cells = p->plane * size_t(g->nz);
+    p->n = int64_t(c->n_steps);
+    p->C = c->ckpt_every ? int64_t(c->ckpt_every) : int64_t(std::ceil(std::sqrt(double(p->n))));
+    if (p->C < 1) p->C = 1;
+    if (p->C > p->n) p->C = p->n;
+    p->S = (p->n + p->C - 1) / p->C;
+    p->src = (long long)((c->src_z * g->ny + c->src_y) * g->nx + c->src_x);
+    p->rec_z = int(c->rec_z);
+    const size_t field = up(p->cells * esz);
+    size_t o = 0;
+    p->ring = o;
+    o += 3 * field;
+    p->lam = o;
+    o += 3 * field;
+    p->ckpt = o;
+    o += size_t(2 * p->S) * field;
+    p->lap = o;
+    o += size_t(p->C) * field;
+    p->traces = o;
+    o += up(size_t(p->n) * p->plane * esz);
+    p->partial = o;
+    o += up(kPartials * sizeof(double));
+    p->misfit = o;
+    o += up(sizeof(double));
+    p->total = o;
+    return true;
+}
+
+template <class T>
+struct Ops;
+template <>
+struct Ops<float> {
+    static hipError_t fwd(const float* p, const float* pp, const float* m, float* pn, float* lap, const Dims& d,
+                          const Scalars<float>& s, long long src, const float* w, int n, hipStream_t st) {
+        return fwi_fwd_step_f32(p, pp, m, pn, lap, d, s, src, w, n, st);
+    }
+    static hipError_t adj(const float* l1, const float* l2, const float* m, const float* lap, float* g, float* lo,
+                          const float* res, int rz, const Dims& d, const Scalars<float>& s, long long src,
+                          float* tr, hipStream_t st) {
+        return fwi_adj_step_f32(l1, l2, m, lap, g, lo, res, rz, d, s, src, tr, st);
+    }
+    static hipError_t residual(const float* a, const float* b, float* r, size_t n, double* part, double* out,
+                               hipStream_t st) {
+        return fwi_residual_f32(a, b, r, n, part, out, st);
+    }
+    static hipError_t fill(float* p, size_t n, hipStream_t st) { return fill_f32(p, n, 0.f, st); }
+};
+template <>
+struct Ops<double> {
+    static hipError_t fwd(const double* p, const double* pp, const double* m, double* pn, double* lap,
+                          const Dims& d, const Scalars<double>& s, long long src, const double* w, int n,
+                          hipStream_t st) {
+        return fwi_fwd_step_f64(p, pp, m, pn, lap, d, s, src, w, n, st);
+    }
+    static hipError_t adj(const double* l1, const double* l2, const double* m, const double* lap, double* g,
+                          double* lo, const double* res, int rz, const Dims& d, const Scalars<double>& s,
+                          long long src, double* tr, hipStream_t st) {
+        return fwi_adj_step_f64(l1, l2, m, lap, g, lo, res, rz, d, s, src, tr, st);
+    }
+    static hipError_t residual(const double* a, const double* b, double* r, size_t n, double* part, double* out,
+                               hipStream_t st) {
+        return fwi_residual_f64(a, b, r, n, part, out, st);
+    }
+    static hipError_t fill(double* p, size_t n, hipStream_t st) { return fill_f64(p, n, 0.0, st); }
+};
+
+#define FWI_TRY(x)                                      \
+    do {                                                \
+        const hipError_t e_ = (x);                      \
+        if (e_ != hipSuccess) return STENCIL_ERR_HIP;   \
+    } while (0)
+
+template <class T>
+T* at(void* work, size_t off) {
+    return reinterpret_cast<T*>(static_cast<char*>(work) + off);
+}
+
+template <class T>
+Scalars<T> scalars(const stencil_grid* g) {
+    return make_scalars<T>(T(g->dx), T(g->dy), T(g->dz), T(g->dt));
+}
+
+template <class T>
+int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, const T* wavelet, const T* d_obs,
+            T* traces_out, double* misfit_out, void* work, hipStream_t st) {
+    Plan p;
+    if (!m || !wavelet || !work || !make_plan(g, c, sizeof(T), &p)) return STENCIL_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
+    const Scalars<T> s = scalars<T>(g);
+    const size_t field = up(p.cells * sizeof(T));
+    T* ring[3] = {at<T>(work, p.ring), at<T>(work, p.ring + field), at<T>(work, p.ring + 2 * field)};
+    T* traces = at<T>(work, p.traces);
+    FWI_TRY(Ops<T>::fill(ring[0], p.cells, st));
+    FWI_TRY(Ops<T>::fill(ring[1], p.cells, st));
+    FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
+    int prev = 0, cur = 1, next = 2;
+    for (int64_t n = 0; n < p.n; ++n) {
+        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, p.d, s, p.src, wavelet, int(n), st));
+        FWI_TRY(hipMemcpyAsync(traces + size_t(n) * p.plane, ring[next] + size_t(p.rec_z) * p.plane,
+                               p.plane * sizeof(T), hipMemcpyDeviceToDevice, st));
+        if (n % p.C == 0) {
+            T* ck = at<T>(work, p.ckpt + size_t(2 * (n / p.C)) * field);
+            FWI_TRY(hipMemcpyAsync(ck, ring[cur], p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
+            FWI_TRY(hipMemcpyAsync(reinterpret_cast<char*>(ck) + field, ring[next], p.cells * sizeof(T),
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        const int t = prev;
+        prev = cur;
+        cur = next;
+        next = t;
+    }
+    const size_t nt = size_t(p.n) * p.plane;
+    if (traces_out)
+        FWI_TRY(hipMemcpyAsync(traces_out, traces, nt * sizeof(T), hipMemcpyDeviceToDevice, st));
+    if (d_obs) {
+        double* mis = at<double>(work, p.misfit);
+        FWI_TRY(Ops<T>::residual(traces, d_obs, traces, nt, at<double>(work, p.partial), mis, st));
+        if (misfit_out)
+            FWI_TRY(hipMemcpyAsync(misfit_out, mis, sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    return STENCIL_OK;
+}
+
+template <class T>
+int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, const T* wavelet, T* grad,
+             T* src_trace, void* work, hipStream_t st) {
+    Plan p;
+    if (!m || !wavelet || !grad || !work || !make_plan(g, c, sizeof(T), &p)) return STENCIL_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
+    const Scalars<T> s = scalars<T>(g);
+    const size_t field = up(p.cells * sizeof(T));
+    T* ring[3] = {at<T>(work, p.ring), at<T>(work, p.ring + field), at<T>(work, p.ring + 2 * field)};
+    T* lam[3] = {at<T>(work, p.lam), at<T>(work, p.lam + field), at<T>(work, p.lam + 2 * field)};
+    const T* res = at<T>(work, p.traces);
+    FWI_TRY(Ops<T>::fill(grad, p.cells, st));
+    for (auto* l : lam) FWI_TRY(Ops<T>::fill(l, p.cells, st));
+    int l1 = 0, l2 = 1, lo = 2;  // lambda^{k+1}, lambda^{k+2}, lambda^k
+    for (int64_t sg = p.S - 1; sg >= 0; --sg) {
+        const int64_t k_lo = sg * p.C + 1;
+        const int64_t k_hi = std::min((sg + 1) * p.C, p.n);
+        // re-run the forward steps of this segment from its checkpoint
+        const T* ck = at<T>(work, p.ckpt + size_t(2 * sg) * field);
+        FWI_TRY(hipMemcpyAsync(ring[0], ck, p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
+        FWI_TRY(hipMemcpyAsync(ring[1], reinterpret_cast<const char*>(ck) + field, p.cells * sizeof(T),
+                               hipMemcpyDeviceToDevice, st));
+        // L p^N would only meet lambda^{N+1} = 0, so the last segment stops at
+        // step N-1 (wavelet has N samples) and its k = N adjoint step skips
+        // the gradient term.
+        int prev = 0, cur = 1, next = 2;
+        for (int64_t j = k_lo; j <= std::min(k_hi, p.n - 1); ++j) {
+            T* lap = at<T>(work, p.lap + size_t(j - k_lo) * field);
+            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, p.d, s, p.src, wavelet, int(j), st));
+            const int t = prev;
+            prev = cur;
+            cur = next;
+            next = t;
+        }
+        for (int64_t k = k_hi; k >= k_lo; --k) {
+            const T* lap = k < p.n ? at<T>(work, p.lap + size_t(k - k_lo) * field) : nullptr;
+            FWI_TRY(Ops<T>::adj(lam[l1], lam[l2], m, lap, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z,
+                                p.d, s, p.src, src_trace ? src_trace + (k - 1) : nullptr, st));
+            const int t = l2;
+            l2 = l1;
+            l1 = lo;
+            lo = t;
+        }
+    }
+    return STENCIL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t stencil_fwi_workspace_bytes(const stencil_grid* grid, const stencil_fwi_config* cfg, int bytes_per_value) {
+    if (bytes_per_value != 4 && bytes_per_value != 8) return 0;
+    Plan p;
+    if (!make_plan(grid, cfg, size_t(bytes_per_value), &p)) return 0;
+    return p.total;
+}
+
+int stencil_fwi_forward_f32(const float* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                            const float* wavelet, const float* d_obs, float* traces_out, double* misfit_out,
+                            void* work, void* stream) {
+    return forward<float>(vel2, grid, cfg, wavelet, d_obs, traces_out, misfit_out, work,
+                          static_cast<hipStream_t>(stream));
+}
+int stencil_fwi_forward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                            const double* wavelet, const double* d_obs, double* traces_out, double* misfit_out,
+                            void* work, void* stream) {
+    return forward<double>(vel2, grid, cfg, wavelet, d_obs, traces_out, misfit_out, work,
+                           static_cast<hipStream_t>(stream));
+}
+int stencil_fwi_backward_f32(const float* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                             const float* wavelet, float* grad, float* src_trace, void* work, void* stream) {
+    return backward<float>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream));
+}
+int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
+                             const double* wavelet, double* grad, double* src_trace, void* work, void* stream) {
+    return backward<double>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+namespace {
+bool step_grid_ok(const stencil_grid* g) {
+    return g && g->nx > 8 && g->ny > 8 && g->nz > 8 && g->nx <= (1u << 20) && g->ny <= (1u << 20) &&
+           g->nz <= (1u << 20) && g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0;
+}
+Dims dims_of_grid(const stencil_grid* g) {
+    Dims d;
+    d.nx = int64_t(g->nx);
+    d.ny = int64_t(g->ny);
+    d.nz = int64_t(g->nz);
+    return d;
+}
+template <class T>
+int fwd_step_api(const T* p, const T* pp, const T* m, T* pn, T* lap_out, const stencil_grid* g, int64_t src,
+                 const T* wavelet, int64_t n, void* stream) {
+    if (!p || !pp || !m || !pn || !step_grid_ok(g)) return STENCIL_ERR_INVALID;
+    if (pn == p || pn == pp || pn == m || (lap_out && (lap_out == p || lap_out == pn))) return STENCIL_ERR_INVALID;
+    const int64_t cells = int64_t(g->nx * g->ny * g->nz);
+    if (src >= cells || (src >= 0 && (!wavelet || n < 0))) return STENCIL_ERR_INVALID;
+    return Ops<T>::fwd(p, pp, m, pn, lap_out, dims_of_grid(g), scalars<T>(g), (long long)src, wavelet, int(n),
+                       static_cast<hipStream_t>(stream)) == hipSuccess
+               ? STENCIL_OK
+               : STENCIL_ERR_HIP;
+}
+template <class T>
+int adj_step_api(const T* l1, const T* l2, const T* m, const T* lap, T* grad, T* lo, const T* res, int64_t rec_z,
+                 const stencil_grid* g, int64_t src, T* trace, void* stream) {
+    if (!l1 || !l2 || !m || !lo || !step_grid_ok(g)) return STENCIL_ERR_INVALID;
+    if (lo == l1 || lo == l2 || lo == m || (lap && !grad)) return STENCIL_ERR_INVALID;
+    if (res && (rec_z < 4 || rec_z + 4 >= int64_t(g->nz))) return STENCIL_ERR_INVALID;
+    const int64_t cells = int64_t(g->nx * g->ny * g->nz);
+    if (trace && (src < 0 || src >= cells)) return STENCIL_ERR_INVALID;
+    return Ops<T>::adj(l1, l2, m, lap, grad, lo, res, int(rec_z), dims_of_grid(g), scalars<T>(g),
+                       trace ? (long long)src : -1LL, trace, static_cast<hipStream_t>(stream)) == hipSuccess
+               ? STENCIL_OK
+               : STENCIL_ERR_HIP;
+}
+}  // namespace
+
+extern "C" {
+int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const float* vel2, float* p_next,
+                             float* lap_out, const stencil_grid* grid, int64_t src_index, const float* wavelet,
+                             int64_t n, void* stream) {
+    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, stream);
+}
+int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const double* vel2, double* p_next,
+                             double* lap_out, const stencil_grid* grid, int64_t src_index, const double* wavelet,
+                             int64_t n, void* stream) {
+    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, stream);
+}
+int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap, float* grad,
+                             float* lam_out, const float* res_plane, int64_t rec_z, const stencil_grid* grid,
+                             int64_t src_index, float* src_value, void* stream) {
+    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, stream);
+}
+int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
+                             double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
+                             const stencil_grid* grid, int64_t src_index, double* src_value, void* stream) {
+    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, stream);
+}
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Adjoint-state FWI gradient through the C-ABI (include/stencil_capi.h,
+stencil_fwi_*), and its torch.autograd binding.
+
+The scheme (no reference counterpart; the reference models forward only): with
+m = vel2 = v^2, a point source of amplitude wavelet[n] added to p^{n+1}, and
+the receiver plane rec_z recorded at every step,
+
+    p^{n+1} = 2 p^n - p^{n-1} + dt^2 m L p^n + wavelet[n] e_src,   p^0 = p^-1 = 0
+    J(m)    = 1/2 sum_k |p^k(rec_z) - d_obs^k|^2
+
+``FwiProblem.misfit`` runs the checkpointed forward pass, ``gradient`` the
+adjoint sweep (dJ/dm on the full grid, zero on the halo).  ``FwiMisfit`` wraps
+the pair as a torch.autograd.Function of vel2, so ``J.backward()`` fills
+``vel2.grad`` (and a velocity v with vel2 = v * v gets dJ/dv = 2 v dJ/dm by
+autograd).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import check, lib, stencil_fwi_config
+from .api import GridDims, _stream
+
+
+def ricker(n_steps: int, dt: float, f_peak: float, t0: float | None = None, amplitude: float = 1.0) -> np.ndarray:
+    """Ricker wavelet (1 - 2 a) exp(-a), a = (pi f (t - t0))^2, sampled at
+    t = n dt; t0 defaults to 1.2 / f (the wavelet starts near zero)."""
+    t0 = 1.2 / f_peak if t0 is None else t0
+    t = np.arange(n_steps, dtype=np.float64) * dt - t0
+    a = (np.pi * f_peak * t) ** 2
+    return amplitude * (1.0 - 2.0 * a) * np.exp(-a)
+
+
+class FwiProblem:
+    """One acquisition (grid, steps, source cell and wavelet, receiver plane)
+    with its device workspace.  ``dtype`` is torch.float32 or torch.float64."""
+
+    def __init__(self, grid: GridDims, n_steps: int, src: tuple[int, int, int], rec_z: int, wavelet,
+                 device, dtype=None, ckpt_every: int = 0):
+        import torch
+
+        self.torch = torch
+        self.grid = grid
+        self.dtype = dtype or torch.float32
+        if self.dtype not in (torch.float32, torch.float64):
+            raise ValueError("FwiProblem dtype must be float32 or float64")
+        self.device = torch.device(device)
+        self.n_steps = int(n_steps)
+        sz, sy, sx = (int(v) for v in src)
+        self.src = (sz, sy, sx)
+        self.rec_z = int(rec_z)
+        self.cfg = stencil_fwi_config(self.n_steps, sx, sy, sz, self.rec_z, int(ckpt_every))
+        self._g = grid.to_c()
+        esz = 4 if self.dtype == torch.float32 else 8
+        nb = int(lib().stencil_fwi_workspace_bytes(C.byref(self._g), C.byref(self.cfg), esz))
+        if nb == 0:
+            raise ValueError("invalid FWI configuration (source / receiver plane must be interior cells)")
+        self.workspace_bytes = nb
+        # 256-byte aligned: torch's caching allocator returns 512-byte aligned blocks
+        self.work = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        assert self.work.data_ptr() % 256 == 0
+        w = torch.as_tensor(np.asarray(wavelet, dtype=np.float64), dtype=self.dtype)
+        if w.numel() != self.n_steps:
+            raise ValueError(f"wavelet has {w.numel()} samples, need n_steps = {self.n_steps}")
+        self.wavelet = w.to(self.device).contiguous()
+        self.misfit_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._have_residual = False
+        self._sfx = "f32" if self.dtype == torch.float32 else "f64"
+
+    @property
+    def ckpt_every(self) -> int:
+        return self.cfg.ckpt_every or int(math.ceil(math.sqrt(self.n_steps)))
+
+    def _check_field(self, t, name):
+        shape = (self.grid.nz, self.grid.ny, self.grid.nx)
+        if tuple(t.shape) != shape or t.dtype != self.dtype or t.device != self.device or not t.is_contiguous():
+            raise ValueError(f"{name}: need a contiguous {self.dtype} tensor of shape {shape} on {self.device}")
+
+    def _check_traces(self, t, name):
+        shape = (self.n_steps, self.grid.ny, self.grid.nx)
+        if tuple(t.shape) != shape or t.dtype != self.dtype or t.device != self.device or not t.is_contiguous():
+            raise ValueError(f"{name}: need a contiguous {self.dtype} tensor of shape {shape} on {self.device}")
+
+    def traces(self, vel2, stream=None):
+        """Forward modelling: the receiver traces (n_steps, ny, nx)."""
+        self._check_field(vel2, "vel2")
+        out = self.torch.empty((self.n_steps, self.grid.ny, self.grid.nx), dtype=self.dtype, device=self.device)
+        fn = getattr(lib(), f"stencil_fwi_forward_{self._sfx}")
+        check(fn(vel2.data_ptr(), C.byref(self._g), C.byref(self.cfg), self.wavelet.data_ptr(), None,
+                 out.data_ptr(), None, self.work.data_ptr(), _stream(stream)), "fwi_forward")
+        self._have_residual = False
+        return out
+
+    def misfit(self, vel2, d_obs, stream=None):
+        """J = 1/2 |traces(vel2) - d_obs|^2 as a 1-element float64 device tensor;
+        keeps the checkpoints and residual for ``gradient``."""
+        self._check_field(vel2, "vel2")
+        self._check_traces(d_obs, "d_obs")
+        fn = getattr(lib(), f"stencil_fwi_forward_{self._sfx}")
+        check(fn(vel2.data_ptr(), C.byref(self._g), C.byref(self.cfg), self.wavelet.data_ptr(), d_obs.data_ptr(),
+                 None, self.misfit_dev.data_ptr(), self.work.data_ptr(), _stream(stream)), "fwi_forward")
+        self._have_residual = True
+        return self.misfit_dev.clone()
+
+    def gradient(self, vel2, stream=None, src_trace: bool = False):
+        """dJ/dvel2 of the last ``misfit`` call (same vel2); with src_trace the
+        adjoint field at the source cell, lambda^{n+1}, is returned as well."""
+        if not self._have_residual:
+            raise RuntimeError("gradient() needs a preceding misfit() on this problem")
+        self._check_field(vel2, "vel2")
+        grad = self.torch.empty_like(vel2)
+        tr = self.torch.empty(self.n_steps, dtype=self.dtype, device=self.device) if src_trace else None
+        fn = getattr(lib(), f"stencil_fwi_backward_{self._sfx}")
+        check(fn(vel2.data_ptr(), C.byref(self._g), C.byref(self.cfg), self.wavelet.data_ptr(), grad.data_ptr(),
+                 tr.data_ptr() if tr is not None else None, self.work.data_ptr(), _stream(stream)),
+              "fwi_backward")
+        return (grad, tr) if src_trace else grad
+
+
+def _autograd_function():
+    import torch
+
+    class FwiMisfit(torch.autograd.Function):
+        """J(vel2) = 1/2 |traces(vel2) - d_obs|^2 with the adjoint-state gradient."""
+
+        @staticmethod
+        def forward(ctx, vel2, problem: FwiProblem, d_obs):
+            v = vel2.detach().contiguous()
+            J = problem.misfit(v, d_obs)
+            ctx.problem = problem
+            ctx.save_for_backward(v)
+            return J.reshape(())
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (v,) = ctx.saved_tensors
+            g = ctx.problem.gradient(v)
+            return grad_out.to(g.dtype) * g, None, None
+
+    return FwiMisfit
+
+
+_FWI_FN = None
+
+
+def fwi_misfit(vel2, problem: FwiProblem, d_obs):
+    """Differentiable misfit J(vel2) (torch.autograd.Function)."""
+    global _FWI_FN
+    if _FWI_FN is None:
+        _FWI_FN = _autograd_function()
+    return _FWI_FN.apply(vel2, problem, d_obs)
+
+
+__all__ = ["ricker", "FwiProblem", "fwi_misfit"]
