@@ -249,6 +249,7 @@ typedef struct stencil_fwi_config {
     uint64_t src_x, src_y, src_z; /* source cell (interior) */
     uint64_t rec_z;               /* receiver plane (interior z), all nx*ny cells */
     uint64_t ckpt_every;          /* segment length; 0 = ceil(sqrt(n_steps)) */
+    uint64_t variant;             /* stencil_variant of the FP32 steps (FP64: naive kernels) */
 } stencil_fwi_config;
 /* Workspace bytes (256-byte aligned base) for this grid, config and value
  * size (4 or 8); 0 if the arguments are invalid. */
@@ -280,17 +281,17 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
  * src_value is non-null. */
 int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const float* vel2, float* p_next,
                              float* lap_out, const stencil_grid* grid, int64_t src_index,
-                             const float* wavelet, int64_t n, void* hip_stream);
+                             const float* wavelet, int64_t n, int variant, void* hip_stream);
 int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const double* vel2,
                              double* p_next, double* lap_out, const stencil_grid* grid, int64_t src_index,
-                             const double* wavelet, int64_t n, void* hip_stream);
+                             const double* wavelet, int64_t n, int variant, void* hip_stream);
 int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap,
                              float* grad, float* lam_out, const float* res_plane, int64_t rec_z,
-                             const stencil_grid* grid, int64_t src_index, float* src_value,
+                             const stencil_grid* grid, int64_t src_index, float* src_value, int variant,
                              void* hip_stream);
 int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
                              double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
-                             const stencil_grid* grid, int64_t src_index, double* src_value,
+                             const stencil_grid* grid, int64_t src_index, double* src_value, int variant,
                              void* hip_stream);
 
 /* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)

@@ -53,7 +53,7 @@ class stencil_field16(C.Structure):
 
 class stencil_fwi_config(C.Structure):
     _fields_ = [("n_steps", C.c_uint64), ("src_x", C.c_uint64), ("src_y", C.c_uint64), ("src_z", C.c_uint64),
-                ("rec_z", C.c_uint64), ("ckpt_every", C.c_uint64)]
+                ("rec_z", C.c_uint64), ("ckpt_every", C.c_uint64), ("variant", C.c_uint64)]
 
 
 class stencil_perf(C.Structure):
@@ -112,10 +112,10 @@ SIGNATURES = {
     "stencil_fwi_forward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P, _P]),
     "stencil_fwi_backward_f32": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
     "stencil_fwi_backward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
-    "stencil_forward_step_f32": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _P]),
-    "stencil_forward_step_f64": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _P]),
-    "stencil_adjoint_step_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _P]),
-    "stencil_adjoint_step_f64": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _P]),
+    "stencil_forward_step_f32": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _I, _P]),
+    "stencil_forward_step_f64": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _I, _P]),
+    "stencil_adjoint_step_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _I, _P]),
+    "stencil_adjoint_step_f64": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _I, _P]),
     "stencil_config_default": (None, [C.POINTER(stencil_config)]),
     "stencil_solver_create": (_I, [C.POINTER(stencil_config), C.POINTER(_P)]),
     "stencil_solver_step": (_I, [_P]),

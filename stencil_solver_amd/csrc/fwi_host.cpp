@@ -50,6 +50,32 @@ constexpr size_t kPartials = 1024;  // residual reduction blocks
 
 size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
+// Source cell and launch choice shared by the step calls of one run.
+struct StepCtx {
+    Dims d;
+    long long src = -1;  // linear cell, -1 = none
+    int src_z = -1, src_y = -1, src_x = -1;
+    bool tiled = false;  // FP32 on the LDS z-queue kernel
+    ZqShape shape{};
+};
+
+StepCtx step_ctx(const Dims& d, long long src) {
+    StepCtx c;
+    c.d = d;
+    c.src = src;
+    if (src >= 0) {
+        const long long plane = d.nx * d.ny;
+        c.src_z = int(src / plane);
+        c.src_y = int((src % plane) / d.nx);
+        c.src_x = int(src % d.nx);
+    }
+    ZqShape tuned;
+    if (zq_cached_shape(d, 0, &tuned)) c.shape = tuned;
+    return c;
+}
+
+bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
+
 struct Plan {
     Dims d;
     size_t cells = 0, plane = 0;
@@ -64,7 +90,7 @@ bool make_plan(const stencil_grid* g, const stencil_fwi_config* c, size_t esz, P
     if (!g || !c || g->nx <= 8 || g->ny <= 8 || g->nz <= 8 || c->n_steps == 0) return false;
     if (g->nx > (1u << 20) || g->ny > (1u << 20) || g->nz > (1u << 20)) return false;
     if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !(g->dt > 0)) return false;
-    if (c->n_steps > (1u << 30)) return false;
+    if (c->n_steps > (1u << 30) || c->variant > STENCIL_VARIANT_MULTIGPU) return false;
     auto inside = [](uint64_t v, uint64_t n) { return v >= 4 && v + 4 < n; };
     if (!inside(c->src_x, g->nx) || !inside(c->src_y, g->ny) || !inside(c->src_z, g->nz)) return false;
     if (!inside(c->rec_z, g->nz)) return false;
@@ -102,16 +128,27 @@ bool make_plan(const stencil_grid* g, const stencil_fwi_config* c, size_t esz, P
 
 template <class T>
 struct Ops;
+// FP32 steps run on the tiled kernel when it takes the dims and pointers
+// (bit-identical to the global-memory kernels), else on the latter.
 template <>
 struct Ops<float> {
-    static hipError_t fwd(const float* p, const float* pp, const float* m, float* pn, float* lap, const Dims& d,
-                          const Scalars<float>& s, long long src, const float* w, int n, hipStream_t st) {
-        return fwi_fwd_step_f32(p, pp, m, pn, lap, d, s, src, w, n, st);
+    static hipError_t fwd(const float* p, const float* pp, const float* m, float* pn, float* lap, const StepCtx& c,
+                          const Scalars<float>& s, const float* w, int n, hipStream_t st) {
+        const void* ptrs[4] = {p, pp, m, pn};
+        if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || aligned16(lap)))
+            return wave_zq_fwi_f32(p, pp, m, pn, lap, c.src_z, c.src_y, c.src_x, c.src >= 0 ? w + n : nullptr, c.d,
+                                   s, c.shape, st);
+        return fwi_fwd_step_f32(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
     }
     static hipError_t adj(const float* l1, const float* l2, const float* m, const float* lap, float* g, float* lo,
-                          const float* res, int rz, const Dims& d, const Scalars<float>& s, long long src,
-                          float* tr, hipStream_t st) {
-        return fwi_adj_step_f32(l1, l2, m, lap, g, lo, res, rz, d, s, src, tr, st);
+                          const float* res, int rz, const StepCtx& c, const Scalars<float>& s, float* tr,
+                          hipStream_t st) {
+        const void* ptrs[4] = {l1, l2, m, lo};
+        if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || (aligned16(lap) && aligned16(g))) &&
+            (!res || aligned16(res)))
+            return adjoint_zq_f32(l1, l2, m, lap, g, lo, res, rz, c.src_z, c.src_y, c.src_x, tr, c.d, s, c.shape,
+                                  st);
+        return fwi_adj_step_f32(l1, l2, m, lap, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
     }
     static hipError_t residual(const float* a, const float* b, float* r, size_t n, double* part, double* out,
                                hipStream_t st) {
@@ -122,14 +159,13 @@ struct Ops<float> {
 template <>
 struct Ops<double> {
     static hipError_t fwd(const double* p, const double* pp, const double* m, double* pn, double* lap,
-                          const Dims& d, const Scalars<double>& s, long long src, const double* w, int n,
-                          hipStream_t st) {
-        return fwi_fwd_step_f64(p, pp, m, pn, lap, d, s, src, w, n, st);
+                          const StepCtx& c, const Scalars<double>& s, const double* w, int n, hipStream_t st) {
+        return fwi_fwd_step_f64(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
     }
     static hipError_t adj(const double* l1, const double* l2, const double* m, const double* lap, double* g,
-                          double* lo, const double* res, int rz, const Dims& d, const Scalars<double>& s,
-                          long long src, double* tr, hipStream_t st) {
-        return fwi_adj_step_f64(l1, l2, m, lap, g, lo, res, rz, d, s, src, tr, st);
+                          double* lo, const double* res, int rz, const StepCtx& c, const Scalars<double>& s,
+                          double* tr, hipStream_t st) {
+        return fwi_adj_step_f64(l1, l2, m, lap, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
     }
     static hipError_t residual(const double* a, const double* b, double* r, size_t n, double* part, double* out,
                                hipStream_t st) {
@@ -156,11 +192,13 @@ Scalars<T> scalars(const stencil_grid* g) {
 
 template <class T>
 int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, const T* wavelet, const T* d_obs,
-            T* traces_out, double* misfit_out, void* work, hipStream_t st) {
+            T* traces_out, double* misfit_out, void* work, hipStream_t st, bool tiled) {
     Plan p;
     if (!m || !wavelet || !work || !make_plan(g, c, sizeof(T), &p)) return STENCIL_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
     const Scalars<T> s = scalars<T>(g);
+    StepCtx cx = step_ctx(p.d, p.src);
+    cx.tiled = tiled;
     const size_t field = up(p.cells * sizeof(T));
     T* ring[3] = {at<T>(work, p.ring), at<T>(work, p.ring + field), at<T>(work, p.ring + 2 * field)};
     T* traces = at<T>(work, p.traces);
@@ -169,7 +207,7 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
     FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
     int prev = 0, cur = 1, next = 2;
     for (int64_t n = 0; n < p.n; ++n) {
-        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, p.d, s, p.src, wavelet, int(n), st));
+        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, cx, s, wavelet, int(n), st));
         FWI_TRY(hipMemcpyAsync(traces + size_t(n) * p.plane, ring[next] + size_t(p.rec_z) * p.plane,
                                p.plane * sizeof(T), hipMemcpyDeviceToDevice, st));
         if (n % p.C == 0) {
@@ -197,11 +235,13 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
 
 template <class T>
 int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, const T* wavelet, T* grad,
-             T* src_trace, void* work, hipStream_t st) {
+             T* src_trace, void* work, hipStream_t st, bool tiled) {
     Plan p;
     if (!m || !wavelet || !grad || !work || !make_plan(g, c, sizeof(T), &p)) return STENCIL_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
     const Scalars<T> s = scalars<T>(g);
+    StepCtx cx = step_ctx(p.d, p.src);
+    cx.tiled = tiled;
     const size_t field = up(p.cells * sizeof(T));
     T* ring[3] = {at<T>(work, p.ring), at<T>(work, p.ring + field), at<T>(work, p.ring + 2 * field)};
     T* lam[3] = {at<T>(work, p.lam), at<T>(work, p.lam + field), at<T>(work, p.lam + 2 * field)};
@@ -223,7 +263,7 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
         int prev = 0, cur = 1, next = 2;
         for (int64_t j = k_lo; j <= std::min(k_hi, p.n - 1); ++j) {
             T* lap = at<T>(work, p.lap + size_t(j - k_lo) * field);
-            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, p.d, s, p.src, wavelet, int(j), st));
+            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(j), st));
             const int t = prev;
             prev = cur;
             cur = next;
@@ -232,7 +272,7 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
         for (int64_t k = k_hi; k >= k_lo; --k) {
             const T* lap = k < p.n ? at<T>(work, p.lap + size_t(k - k_lo) * field) : nullptr;
             FWI_TRY(Ops<T>::adj(lam[l1], lam[l2], m, lap, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z,
-                                p.d, s, p.src, src_trace ? src_trace + (k - 1) : nullptr, st));
+                                cx, s, src_trace ? src_trace + (k - 1) : nullptr, st));
             const int t = l2;
             l2 = l1;
             l1 = lo;
@@ -257,21 +297,23 @@ int stencil_fwi_forward_f32(const float* vel2, const stencil_grid* grid, const s
                             const float* wavelet, const float* d_obs, float* traces_out, double* misfit_out,
                             void* work, void* stream) {
     return forward<float>(vel2, grid, cfg, wavelet, d_obs, traces_out, misfit_out, work,
-                          static_cast<hipStream_t>(stream));
+                          static_cast<hipStream_t>(stream), cfg && cfg->variant != STENCIL_VARIANT_NAIVE);
 }
 int stencil_fwi_forward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
                             const double* wavelet, const double* d_obs, double* traces_out, double* misfit_out,
                             void* work, void* stream) {
     return forward<double>(vel2, grid, cfg, wavelet, d_obs, traces_out, misfit_out, work,
-                           static_cast<hipStream_t>(stream));
+                           static_cast<hipStream_t>(stream), false);
 }
 int stencil_fwi_backward_f32(const float* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
                              const float* wavelet, float* grad, float* src_trace, void* work, void* stream) {
-    return backward<float>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream));
+    return backward<float>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream),
+                           cfg && cfg->variant != STENCIL_VARIANT_NAIVE);
 }
 int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
                              const double* wavelet, double* grad, double* src_trace, void* work, void* stream) {
-    return backward<double>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream));
+    return backward<double>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream),
+                            false);
 }
 
 }  // extern "C"
@@ -288,28 +330,33 @@ Dims dims_of_grid(const stencil_grid* g) {
     d.nz = int64_t(g->nz);
     return d;
 }
+bool variant_ok(int v) { return v >= STENCIL_VARIANT_NAIVE && v <= STENCIL_VARIANT_MULTIGPU; }
 template <class T>
 int fwd_step_api(const T* p, const T* pp, const T* m, T* pn, T* lap_out, const stencil_grid* g, int64_t src,
-                 const T* wavelet, int64_t n, void* stream) {
-    if (!p || !pp || !m || !pn || !step_grid_ok(g)) return STENCIL_ERR_INVALID;
+                 const T* wavelet, int64_t n, int variant, void* stream) {
+    if (!p || !pp || !m || !pn || !step_grid_ok(g) || !variant_ok(variant)) return STENCIL_ERR_INVALID;
     if (pn == p || pn == pp || pn == m || (lap_out && (lap_out == p || lap_out == pn))) return STENCIL_ERR_INVALID;
     const int64_t cells = int64_t(g->nx * g->ny * g->nz);
     if (src >= cells || (src >= 0 && (!wavelet || n < 0))) return STENCIL_ERR_INVALID;
-    return Ops<T>::fwd(p, pp, m, pn, lap_out, dims_of_grid(g), scalars<T>(g), (long long)src, wavelet, int(n),
-                       static_cast<hipStream_t>(stream)) == hipSuccess
+    StepCtx cx = step_ctx(dims_of_grid(g), src >= 0 ? (long long)src : -1LL);
+    cx.tiled = variant != STENCIL_VARIANT_NAIVE;
+    return Ops<T>::fwd(p, pp, m, pn, lap_out, cx, scalars<T>(g), wavelet, int(n), static_cast<hipStream_t>(stream)) ==
+                   hipSuccess
                ? STENCIL_OK
                : STENCIL_ERR_HIP;
 }
 template <class T>
 int adj_step_api(const T* l1, const T* l2, const T* m, const T* lap, T* grad, T* lo, const T* res, int64_t rec_z,
-                 const stencil_grid* g, int64_t src, T* trace, void* stream) {
-    if (!l1 || !l2 || !m || !lo || !step_grid_ok(g)) return STENCIL_ERR_INVALID;
+                 const stencil_grid* g, int64_t src, T* trace, int variant, void* stream) {
+    if (!l1 || !l2 || !m || !lo || !step_grid_ok(g) || !variant_ok(variant)) return STENCIL_ERR_INVALID;
     if (lo == l1 || lo == l2 || lo == m || (lap && !grad)) return STENCIL_ERR_INVALID;
     if (res && (rec_z < 4 || rec_z + 4 >= int64_t(g->nz))) return STENCIL_ERR_INVALID;
     const int64_t cells = int64_t(g->nx * g->ny * g->nz);
     if (trace && (src < 0 || src >= cells)) return STENCIL_ERR_INVALID;
-    return Ops<T>::adj(l1, l2, m, lap, grad, lo, res, int(rec_z), dims_of_grid(g), scalars<T>(g),
-                       trace ? (long long)src : -1LL, trace, static_cast<hipStream_t>(stream)) == hipSuccess
+    StepCtx cx = step_ctx(dims_of_grid(g), trace ? (long long)src : -1LL);
+    cx.tiled = variant != STENCIL_VARIANT_NAIVE;
+    return Ops<T>::adj(l1, l2, m, lap, grad, lo, res, int(rec_z), cx, scalars<T>(g), trace,
+                       static_cast<hipStream_t>(stream)) == hipSuccess
                ? STENCIL_OK
                : STENCIL_ERR_HIP;
 }
@@ -318,22 +365,22 @@ int adj_step_api(const T* l1, const T* l2, const T* m, const T* lap, T* grad, T*
 extern "C" {
 int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const float* vel2, float* p_next,
                              float* lap_out, const stencil_grid* grid, int64_t src_index, const float* wavelet,
-                             int64_t n, void* stream) {
-    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, stream);
+                             int64_t n, int variant, void* stream) {
+    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, variant, stream);
 }
 int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const double* vel2, double* p_next,
                              double* lap_out, const stencil_grid* grid, int64_t src_index, const double* wavelet,
-                             int64_t n, void* stream) {
-    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, stream);
+                             int64_t n, int variant, void* stream) {
+    return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, variant, stream);
 }
 int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap, float* grad,
                              float* lam_out, const float* res_plane, int64_t rec_z, const stencil_grid* grid,
-                             int64_t src_index, float* src_value, void* stream) {
-    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, stream);
+                             int64_t src_index, float* src_value, int variant, void* stream) {
+    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
 }
 int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
                              double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
-                             const stencil_grid* grid, int64_t src_index, double* src_value, void* stream) {
-    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, stream);
+                             const stencil_grid* grid, int64_t src_index, double* src_value, int variant, void* stream) {
+    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
 }
 }  // extern "C"

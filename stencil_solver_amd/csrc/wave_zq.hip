@@ -34,7 +34,11 @@ using dev::comp;
 using dev::Fd;
 using dev::kR;
 
-enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3 };
+// kWaveFwi: the exact wave step plus a point source and an optional L p
+// output (FWI forward / recompute); kAdjoint: the adjoint step
+// lambda_out = 2 lambda1 - lambda2 + dt^2 L(m lambda1) with the residual on
+// the receiver plane and the gradient update (FWI backward).
+enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3, kWaveFwi = 4, kAdjoint = 5 };
 
 struct ZqArgs {
     const void* cur;
@@ -53,6 +57,20 @@ struct ZqArgs {
     Scalars<float> s;
     float coef;                      // heat: dt2 * alpha
     float w0, wx[4], wy[4], wz[4];   // fused weights (heat: pre-multiplied by coef)
+    // FWI modes (FP32).  kWaveFwi: cur = p^n, prev = p^{n-1}, vel2 = m,
+    // next = p^{n+1}, lap_out = L p^n (nullable), src_val = &wavelet[n]
+    // (nullable) added at cell (src_z, src_y, src_x).  kAdjoint: cur =
+    // lambda^{k+1}, aux = m, prev = lambda^{k+2}, vel2 = L p^k (nullable: no
+    // gradient term), grad updated in place, next = lambda^k, res = residual
+    // plane (nullable) added on plane rec_z, trace (nullable) = lambda^k at
+    // the source cell.
+    const void* aux;
+    void* lap_out;
+    void* grad;
+    const float* res;
+    const float* src_val;
+    float* trace;
+    int rec_z, src_z, src_y, src_x;
 };
 
 using dev::Fmt;
@@ -144,18 +162,22 @@ __device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned 
 // where the kernel fits without scratch; 3 (168 VGPRs) for a 768-thread
 // workgroup, and where 128 would spill: three halo items per thread (64x4),
 // or two with block-FP16 storage (exponent registers).
-constexpr int min_waves(int txv, int ty, int fmt) {
+constexpr int min_waves(int txv, int ty, int fmt, int mode) {
     const int threads = txv * ty;
     const int halo_per = (2 * dev::kR * txv + 2 * ty + threads - 1) / threads;
     if (threads == 768 || halo_per >= 3 || (halo_per == 2 && fmt == dev::kBfp16)) return 3;
+    if (mode == kAdjoint && threads != 1024) return 3;  // m and lambda staged, three more streams
     return 4;
 }
 
 template <int TXV, int TY, int MODE, bool NT, int FMT = dev::kF32>
-__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, FMT))))
+__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, FMT, MODE))))
 void zq_kernel(ZqArgs a) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
-    constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact);
+    constexpr bool kFwd = (MODE == kWaveFwi);
+    constexpr bool kAdj = (MODE == kAdjoint);
+    constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact || kFwd || kAdj);
+    static_assert(!(kFwd || kAdj) || FMT == dev::kF32, "FWI modes are FP32");
     constexpr bool kBlk = (FMT == dev::kBfp16);
     constexpr unsigned kEsz = FMT == dev::kF32 ? 4u : 2u;  // bytes per stored value
     constexpr int kThreads = TXV * TY;
@@ -263,7 +285,10 @@ void zq_kernel(ZqArgs a) {
     // holds plane zz (x and y neighbours), zq[k] is plane zz - 4 + k.
     // dt2v multiplies vv in the fused forms (dt^2, or dt^2 * 2^k when vv holds
     // the raw mantissas of a block-FP16 vel2 block).
-    auto compute = [&](int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v) -> float4 {
+    // lc: lambda^{k+1} at the point (kAdjoint; the staged values are m lambda);
+    // lap4 receives L p (kWaveFwi).
+    auto compute = [&](int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v, float4 lc,
+                       float4& lap4) -> float4 {
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ctr4 = zq[kR];
         const float4 xl = lds[b][ty + kR][tx];
@@ -291,8 +316,14 @@ void zq_kernel(ZqArgs a) {
                     ly += Fd<float>::c(r) * (comp(yp[r - 1], i) + comp(ym[r - 1], i)) * a.s.inv_dy2;
                     lz += Fd<float>::c(r) * (comp(zq[kR + r], i) + comp(zq[kR - r], i)) * a.s.inv_dz2;
                 }
-                if constexpr (kHeat) res = p + a.coef * (lx + ly + lz);
-                else res = 2.f * p - comp(pv, i) + a.s.dt2 * comp(vv, i) * (lx + ly + lz);
+                if constexpr (kHeat) {
+                    res = p + a.coef * (lx + ly + lz);
+                } else if constexpr (kAdj) {
+                    res = 2.f * comp(lc, i) - comp(pv, i) + a.s.dt2 * (lx + ly + lz);
+                } else {
+                    res = 2.f * p - comp(pv, i) + a.s.dt2 * comp(vv, i) * (lx + ly + lz);
+                    if constexpr (kFwd) dev::set_comp(lap4, i, lx + ly + lz);
+                }
             } else {
                 // Combined weights w_r = c_r / h^2, one accumulator.
                 float acc = kHeat ? __builtin_fmaf(a.w0, p, p) : a.w0 * p;
@@ -315,61 +346,134 @@ void zq_kernel(ZqArgs a) {
     };
     // Steps past z_end (the tail of the last group of 9 in a chunk whose length
     // is not a multiple of 9) run on clamped planes and store nothing.
-    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v) {
+    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 pv, float4 vv, float dt2v, float4 lc,
+                      float4 gr) {
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own_out) out = compute(b, zq, pv, vv, dt2v);
-        if (zz < z_end)
+        float4 lap4 = out;
+        if (own_out) out = compute(b, zq, pv, vv, dt2v, lc, lap4);
+        if (zz < z_end) {
+            if constexpr (kFwd || kAdj) {
+                // the source / trace cell lies in this lane's 4 cells of this plane
+                const bool at_src = own_out && zz == a.src_z && y == a.src_y && a.src_x >= x && a.src_x < x + 4;
+                if constexpr (kFwd) {
+                    if (at_src && a.src_val) dev::set_comp(out, a.src_x - x, comp(out, a.src_x - x) + *a.src_val);
+                    if (a.lap_out)
+                        store4<FMT, NT, true>(plane_rsrc(a.lap_out, zz, plane_bytes), plane_rsrc(a.lap_out, zz, 0),
+                                              own_b, 0, lap4, 1.f, own_out, false);
+                } else {
+                    if (a.res && zz == a.rec_z && own_out) {
+                        const float4 r = *reinterpret_cast<const float4*>(a.res + own_off);
+                        out = make_float4(out.x + r.x, out.y + r.y, out.z + r.z, out.w + r.w);
+                    }
+                    if (a.vel2) {  // gradient: grad += dt^2 * (L p^k) * lambda^{k+1}
+                        const float4 g = make_float4(gr.x + a.s.dt2 * vv.x * lc.x, gr.y + a.s.dt2 * vv.y * lc.y,
+                                                     gr.z + a.s.dt2 * vv.z * lc.z, gr.w + a.s.dt2 * vv.w * lc.w);
+                        store4<FMT, NT, true>(plane_rsrc(a.grad, zz, plane_bytes), plane_rsrc(a.grad, zz, 0),
+                                              own_b, 0, g, 1.f, own_out, false);
+                    }
+                    if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x);
+                }
+            }
             store4<FMT, NT, kExact>(plane_rsrc(next_b, zz, plane_bytes),
                                     plane_rsrc(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
                                     a.p_scale, own_out, blk_out);
+        }
     };
 
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
+    const char* const aux_b = static_cast<const char*>(a.aux);
+    // kAdjoint stages m * lambda (the product the adjoint Laplacian acts on).
+    auto staged = [&](int z, unsigned boff) -> float4 {
+        if constexpr (FMT == dev::kF32) {
+            const float4 l = raw_at(cur_b, z, boff);
+            if constexpr (kAdj) {
+                const float4 m = raw_at(aux_b, z, boff);
+                return make_float4(m.x * l.x, m.y * l.y, m.z * l.z, m.w * l.w);
+            }
+            return l;
+        } else {
+            return make_float4(0.f, 0.f, 0.f, 0.f);  // 16-bit storage decodes in its own loop
+        }
+    };
     float4 q[9];
+    if constexpr (FMT == dev::kF32) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        q[k] = decode<FMT, kExact>(raw_at(cur_b, z_begin - kR + k, own_ld),
-                                   exp_at(a.cur_e, z_begin - kR + k, own_eld), p_inv);
+        for (int k = 0; k < 8; ++k) q[k] = staged(z_begin - kR + k, own_ld);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            q[k] = decode<FMT, kExact>(raw_at(cur_b, z_begin - kR + k, own_ld),
+                                       exp_at(a.cur_e, z_begin - kR + k, own_eld), p_inv);
+    }
 
     if constexpr (FMT == dev::kF32) {
         // FP32: the queue refill (plane zz+4) is loaded at the top of step zz;
         // halo and streams for plane zz+1 are prefetched one step ahead.
-        float4 h_cur[kHaloPer];
+        // (kAdjoint also prefetches m for the halo, lambda1 at the point, and
+        // the gradient; its vel2 stream is L p^k, absent when vel2 is null.)
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool lap_in = kAdj && a.vel2 != nullptr;
+        float4 h_cur[kHaloPer], hm_cur[kHaloPer];
 #pragma unroll
-        for (int k = 0; k < kHaloPer; ++k) h_cur[k] = raw_at(cur_b, z_begin, h_ld[k]);
-        float4 pv_cur = make_float4(0.f, 0.f, 0.f, 0.f), vv_cur = pv_cur;
+        for (int k = 0; k < kHaloPer; ++k) {
+            h_cur[k] = raw_at(cur_b, z_begin, h_ld[k]);
+            if constexpr (kAdj) hm_cur[k] = raw_at(aux_b, z_begin, h_ld[k]);
+        }
+        float4 pv_cur = zero4, vv_cur = zero4, lc_cur = zero4, gr_cur = zero4;
         if constexpr (!kHeat) {
             pv_cur = raw_stream(prev_b, z_begin);
-            vv_cur = raw_stream(vel_b, z_begin);
+            if (!kAdj || lap_in) vv_cur = raw_stream(vel_b, z_begin);
+        }
+        if constexpr (kAdj) {
+            lc_cur = raw_at(cur_b, z_begin, own_ld);
+            if (lap_in) gr_cur = raw_stream(static_cast<const char*>(a.grad), z_begin);
         }
         for (int zb = z_begin; zb < z_end; zb += 9) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
                 const int zz = zb + j;
                 const int z1 = min(zz + 1, z_last);
-                q[(j + 8) % 9] = raw_at(cur_b, min(zz + kR, z_last), own_ld);
-                float4 h_nxt[kHaloPer];
+                q[(j + 8) % 9] = staged(min(zz + kR, z_last), own_ld);
+                float4 h_nxt[kHaloPer], hm_nxt[kHaloPer];
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) h_nxt[k] = raw_at(cur_b, z1, h_ld[k]);
-                float4 pv_nxt = make_float4(0.f, 0.f, 0.f, 0.f), vv_nxt = pv_nxt;
+                for (int k = 0; k < kHaloPer; ++k) {
+                    h_nxt[k] = raw_at(cur_b, z1, h_ld[k]);
+                    if constexpr (kAdj) hm_nxt[k] = raw_at(aux_b, z1, h_ld[k]);
+                }
+                float4 pv_nxt = zero4, vv_nxt = zero4, lc_nxt = zero4, gr_nxt = zero4;
                 if constexpr (!kHeat) {
                     pv_nxt = raw_stream(prev_b, z1);
-                    vv_nxt = raw_stream(vel_b, z1);
+                    if (!kAdj || lap_in) vv_nxt = raw_stream(vel_b, z1);
+                }
+                if constexpr (kAdj) {
+                    lc_nxt = raw_at(cur_b, z1, own_ld);
+                    if (lap_in) gr_nxt = raw_stream(static_cast<const char*>(a.grad), z1);
                 }
                 const int b = (zz - z_begin) & 1;
                 lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
-                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = h_cur[k];
+                    if (tid + k * kThreads < kHalo) {
+                        float4 hv = h_cur[k];
+                        if constexpr (kAdj)
+                            hv = make_float4(hm_cur[k].x * hv.x, hm_cur[k].y * hv.y, hm_cur[k].z * hv.z,
+                                             hm_cur[k].w * hv.w);
+                        lds[b][h_row[k]][h_col[k]] = hv;
+                    }
                 __syncthreads();
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv_cur, vv_cur, a.s.dt2);
+                finish(zz, b, zq, pv_cur, vv_cur, a.s.dt2, lc_cur, gr_cur);
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) h_cur[k] = h_nxt[k];
+                for (int k = 0; k < kHaloPer; ++k) {
+                    h_cur[k] = h_nxt[k];
+                    if constexpr (kAdj) hm_cur[k] = hm_nxt[k];
+                }
                 pv_cur = pv_nxt;
                 vv_cur = vv_nxt;
+                lc_cur = lc_nxt;
+                gr_cur = gr_nxt;
             }
         }
     } else {
@@ -437,7 +541,7 @@ void zq_kernel(ZqArgs a) {
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv, vv, dt2v);
+                finish(zz, b, zq, pv, vv, dt2v, pv, pv);
             }
         }
     }
@@ -466,6 +570,8 @@ KernelFn pick_shape(int txv, int ty) {
 
 KernelFn pick(int txv, int ty, int mode, bool nt) {
     switch (mode) {
+        case kWaveFwi: return pick_shape<kWaveFwi, true>(txv, ty);  // FWI modes: non-temporal streams only
+        case kAdjoint: return pick_shape<kAdjoint, true>(txv, ty);
         case kWaveFused: return nt ? pick_shape<kWaveFused, true>(txv, ty) : pick_shape<kWaveFused, false>(txv, ty);
         case kWaveExact: return nt ? pick_shape<kWaveExact, true>(txv, ty) : pick_shape<kWaveExact, false>(txv, ty);
         case kHeatFused: return nt ? pick_shape<kHeatFused, true>(txv, ty) : pick_shape<kHeatFused, false>(txv, ty);
@@ -631,6 +737,53 @@ hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Fi
         fill_weights(a, 1.0);
     }
     return launch_zq(a, d, shape, exact ? kWaveExact : kWaveFused, st, fmt);
+}
+
+hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, float* pn, float* lap_out,
+                           int src_z, int src_y, int src_x, const float* src_val, const Dims& d,
+                           const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+    ZqArgs a{};
+    a.cur = p;
+    a.prev = pp;
+    a.vel2 = m;
+    a.next = pn;
+    a.lap_out = lap_out;
+    a.src_val = src_val;
+    a.src_z = src_z;
+    a.src_y = src_y;
+    a.src_x = src_x;
+    a.s = s;
+    shape.nt = 1;
+    return launch_zq(a, d, shape, kWaveFwi, st);
+}
+
+hipError_t adjoint_zq_f32(const float* lam1, const float* lam2, const float* m, const float* lap, float* grad,
+                          float* lam_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+                          float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+    ZqArgs a{};
+    a.cur = lam1;
+    a.aux = m;
+    a.prev = lam2;
+    a.vel2 = lap;
+    a.grad = grad;
+    a.next = lam_out;
+    a.res = res;
+    a.rec_z = rec_z;
+    a.trace = trace;
+    a.src_z = src_z;
+    a.src_y = src_y;
+    a.src_x = src_x;
+    a.s = s;
+    shape.nt = 1;
+    // 64 x 16 (1024 threads, 128 VGPRs) and 64 x 4 (three halo items per
+    // thread) spill with the adjoint's extra streams: use 64 x 8 instead.
+    shape = zq_resolve(d, shape, 0);
+    if (shape.txv == 64 && (shape.ty == 16 || shape.ty == 4)) {
+        shape.ty = 8;
+        shape.n_chunks = 0;
+        shape = zq_resolve(d, shape, 0);
+    }
+    return launch_zq(a, d, shape, kAdjoint, st);
 }
 
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,

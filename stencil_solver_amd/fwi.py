@@ -36,10 +36,13 @@ def ricker(n_steps: int, dt: float, f_peak: float, t0: float | None = None, ampl
 
 class FwiProblem:
     """One acquisition (grid, steps, source cell and wavelet, receiver plane)
-    with its device workspace.  ``dtype`` is torch.float32 or torch.float64."""
+    with its device workspace.  ``dtype`` is torch.float32 or torch.float64;
+    ``variant`` picks the FP32 step kernels ("tiled": the LDS z-queue kernel,
+    "naive": the global-memory one; both give bit-identical results, FP64
+    always uses the latter)."""
 
     def __init__(self, grid: GridDims, n_steps: int, src: tuple[int, int, int], rec_z: int, wavelet,
-                 device, dtype=None, ckpt_every: int = 0):
+                 device, dtype=None, ckpt_every: int = 0, variant: str = "tiled"):
         import torch
 
         self.torch = torch
@@ -52,7 +55,11 @@ class FwiProblem:
         sz, sy, sx = (int(v) for v in src)
         self.src = (sz, sy, sx)
         self.rec_z = int(rec_z)
-        self.cfg = stencil_fwi_config(self.n_steps, sx, sy, sz, self.rec_z, int(ckpt_every))
+        if variant not in ("naive", "tiled"):
+            raise ValueError("variant must be 'naive' or 'tiled'")
+        self.variant = variant
+        self.cfg = stencil_fwi_config(self.n_steps, sx, sy, sz, self.rec_z, int(ckpt_every),
+                                      0 if variant == "naive" else 1)
         self._g = grid.to_c()
         esz = 4 if self.dtype == torch.float32 else 8
         nb = int(lib().stencil_fwi_workspace_bytes(C.byref(self._g), C.byref(self.cfg), esz))
