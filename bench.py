@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--precision-n", type=int, nargs="*", default=None,
                     help="grid sizes of the 16-bit storage study (default: --n)")
     ap.add_argument("--no-fwi", action="store_true", help="skip the adjoint FWI gradient timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-shot timing")
+    ap.add_argument("--batch-shots", type=int, default=64, help="shots of the batched modelling")
+    ap.add_argument("--batch-steps", type=int, default=1500, help="steps of the batched modelling")
     ap.add_argument("--fwi-n", type=int, default=256, help="FWI grid cells per axis")
     ap.add_argument("--fwi-steps", type=int, default=2000, help="FWI time steps")
     ap.add_argument("--calibrate", action="store_true",
@@ -282,6 +285,8 @@ def main():
 
     if not args.no_fwi:
         result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev)
+    if not args.no_batch:
+        result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev)
 
     if args.calibrate:
         # Known-size copy: n^3 floats read + n^3 floats written per launch, the
@@ -397,6 +402,57 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     print(f"[bench] precision {n}^3 x {steps}: " +
           ", ".join(f"{k}: {v.get('gpts', '-')} Gpts/s rel {v.get('rel_l2_vs_fp64', 0):.2e}" for k, v in out.items()
                     if k != "fp64_ref"), file=sys.stderr, flush=True)
+    return out
+
+
+def batch_study(ss, n_shots, steps, dev):
+    """Batched multi-shot modelling: n_shots cubic grids with n drawn uniformly
+    from {96, 128, 160, 192, 256, 320} (seed 4); each shot: seeded smooth
+    velocity (Gaussian-filtered noise, sigma 8 cells) mapped to [1500, 4500]
+    m/s, h = 10 m, dt = 8e-4 s, 15 Hz Ricker at a random interior cell; FP32.
+    Launch shapes autotuned per grid size; the run is one hipGraph launch."""
+    import numpy as np
+    import torch
+    from stencil_solver_amd import synth
+
+    rng = np.random.default_rng(4)
+    sizes = rng.choice([96, 128, 160, 192, 256, 320], size=n_shots)
+    w = ss.ricker(steps, 8e-4, 15.0)
+    shots = []
+    t0 = time.time()
+    for i, n in enumerate(sizes):
+        n = int(n)
+        v = synth.smooth_velocity((n, n, n), 8.0, 1500.0, 4500.0, seed=1000 + i, device=dev)
+        src = tuple(int(c) for c in rng.integers(4, n - 4, size=3))
+        shots.append({"grid": ss.GridDims.cube(n, 10.0, 8e-4), "vel2": (v * v).contiguous(), "wavelet": w,
+                      "src": src})
+        del v
+    t_inputs = time.time() - t0
+    t0 = time.time()
+    b = ss.ShotBatch(shots, steps, dev, tune=True)
+    torch.cuda.synchronize()
+    t_build = time.time() - t0
+    b.run()  # warm-up replay
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    pts = sum((int(n) - 8) ** 3 for n in sizes)
+    gpts = pts * steps / (ms * 1e-3) / 1e9
+    gbs = 16.0 * pts * steps / (ms * 1e-3) / 1e9
+    sizes_hist = {int(k): int(v) for k, v in zip(*np.unique(sizes, return_counts=True))}
+    out = {"shots": n_shots, "steps": steps, "sizes": sizes_hist, "interior_points_per_step": pts,
+           "run_ms": round(ms, 1), "ms_per_step": round(ms / steps, 4), "gpts": round(gpts, 1),
+           "achieved_gbs": round(gbs, 1), "graph_build_s": round(t_build, 2), "inputs_s": round(t_inputs, 2),
+           "groups": b.groups()}
+    print(f"[bench] batch {n_shots} shots x {steps}: {ms:.0f} ms, {gpts:.1f} Gpts/s, {gbs:.0f} GB/s "
+          f"(graph build {t_build:.1f}s)", file=sys.stderr, flush=True)
+    b.close()
+    del b, shots
+    torch.cuda.empty_cache()
     return out
 
 

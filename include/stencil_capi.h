@@ -294,6 +294,36 @@ int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const doubl
                              const stencil_grid* grid, int64_t src_index, double* src_value, int variant,
                              void* hip_stream);
 
+/* ---- Batched multi-shot forward modelling (no reference counterpart).
+ * Each shot is the FWI forward scheme on its own grid: three caller-owned
+ * time-level buffers, vel2, and an optional point source (wavelet of n_steps
+ * device samples; NULL = none).  Shots are grouped by grid; each group's
+ * launch shape comes from the wave autotuner (run at create time when tune is
+ * non-zero, else its cache or the default); the whole run -- zeroing all
+ * levels, then n_steps steps of every group, one launch per group per step --
+ * is captured into one hipGraph at create time.  run() is one graph launch;
+ * afterwards p^{n_steps} of every shot is in ring[stencil_batch_newest()].
+ * Every run starts again from zero fields.  Needs nx % 4 == 0 and 16-byte
+ * aligned buffers. ---- */
+typedef struct stencil_shot {
+    stencil_grid grid;
+    float* ring[3];
+    const float* vel2;
+    const float* wavelet;         /* n_steps samples or NULL */
+    uint64_t src_x, src_y, src_z; /* interior cell (when wavelet is set) */
+} stencil_shot;
+typedef struct stencil_batch stencil_batch;
+int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, int tune,
+                         stencil_batch** out, void* hip_stream);
+int stencil_batch_run(stencil_batch* batch, void* hip_stream);
+/* Ring index holding the newest level (same for every shot). */
+int stencil_batch_newest(const stencil_batch* batch, int shot);
+/* Groups of equal grids, and per group its grid, launch shape and shot count. */
+int stencil_batch_groups(const stencil_batch* batch);
+int stencil_batch_group(const stencil_batch* batch, int group, stencil_grid* grid,
+                        stencil_tile_shape* shape, int* n_shots);
+void stencil_batch_destroy(stencil_batch* batch);
+
 /* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)
  * for callers that cannot use the C++ API. ---- */
 typedef struct stencil_config {

@@ -81,6 +81,7 @@ class Oracle:
             g = getattr(L, f"oracle_fwi_gradient_{suf}")
             g.argtypes = [_P] + [_SZ] * 3 + [T] * 4 + [C.c_int, _SZ, _SZ, _P, _P, _P, _P]
             g.restype = _D
+        L.oracle_shot_run_f32.argtypes = [_P] + [_SZ] * 3 + [_F] * 4 + [C.c_int, _SZ, _P, _P]
 
     # -- coefficients ------------------------------------------------------
     def fd_coeffs(self, dtype=np.float32) -> np.ndarray:
@@ -219,6 +220,18 @@ class Oracle:
         si = (src[0] * ny + src[1]) * nx + src[2]
         J = f(_ptr(m), nx, ny, nz, h, h, h, dt, n_steps, si, rec_z, _ptr(w), _ptr(d), _ptr(grad), _ptr(st))
         return J, grad, st
+
+    def shot_run(self, m, h, dt, n_steps, src, wavelet):
+        """p^{n_steps} of one shot (FP32) from zero fields; src (z, y, x) or None."""
+        nz, ny, nx = m.shape
+        out = np.zeros_like(m)
+        if wavelet is None:
+            self.lib.oracle_shot_run_f32(_ptr(m), nx, ny, nz, h, h, h, dt, n_steps, 0, None, _ptr(out))
+        else:
+            w = np.ascontiguousarray(wavelet, dtype=np.float32)
+            si = (src[0] * ny + src[1]) * nx + src[2]
+            self.lib.oracle_shot_run_f32(_ptr(m), nx, ny, nz, h, h, h, dt, n_steps, si, _ptr(w), _ptr(out))
+        return out
 
     # -- solver runs -------------------------------------------------------
     def wave_run(self, p0, vel2, dx, dt, n_steps, dy=None, dz=None):

@@ -588,3 +588,23 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
 
 ORACLE_FWI(float, f32, oracle_fd_coeffs_f32)
 ORACLE_FWI(double, f64, oracle_fd_coeffs_f64)
+
+/* One shot of the batched modelling: the FWI forward scheme from zero fields,
+ * out = p^{n_steps} (the same levels as oracle_fwi_forward_f32). */
+void oracle_shot_run_f32(const float* m, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                         float dt, int n_steps, size_t src, const float* wavelet, float* out) {
+    const size_t n = nx * ny * nz;
+    float* levels = (float*)malloc((size_t)(n_steps + 1) * n * sizeof(float));
+    float* laps = (float*)malloc((size_t)(n_steps + 1) * n * sizeof(float));
+    float* zero_w = NULL;
+    if (!wavelet) {
+        zero_w = (float*)calloc((size_t)n_steps + 1, sizeof(float));
+        src = (size_t)-1;
+    }
+    fwi_levels_f32(m, (int)nx, (int)ny, (int)nz, dx, dy, dz, dt, n_steps, src, wavelet ? wavelet : zero_w,
+                   levels, laps);
+    memcpy(out, levels + (size_t)n_steps * n, n * sizeof(float));
+    free(levels);
+    free(laps);
+    free(zero_w);
+}

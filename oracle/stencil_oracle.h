@@ -145,6 +145,10 @@ double oracle_fwi_gradient_f64(const double* m, size_t nx, size_t ny, size_t nz,
                                double dy, double dz, double dt, int n_steps, size_t src,
                                size_t rec_z, const double* wavelet, const double* d_obs,
                                double* grad, double* src_trace);
+/* One shot of the batched modelling: p^{n_steps} of the scheme above from
+ * zero fields (wavelet NULL: no source). */
+void oracle_shot_run_f32(const float* m, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                         float dt, int n_steps, size_t src, const float* wavelet, float* out);
 
 #ifdef __cplusplus
 }

@@ -38,4 +38,5 @@ from .api import (  # noqa: F401
     wave_bytes_per_step,
 )
 from ._lib import LIB_PATH, lib  # noqa: F401
+from .batch import ShotBatch  # noqa: F401
 from .fwi import FwiProblem, fwi_misfit, ricker  # noqa: F401

@@ -56,6 +56,11 @@ class stencil_fwi_config(C.Structure):
                 ("rec_z", C.c_uint64), ("ckpt_every", C.c_uint64), ("variant", C.c_uint64)]
 
 
+class stencil_shot(C.Structure):
+    _fields_ = [("grid", stencil_grid), ("ring", C.c_void_p * 3), ("vel2", C.c_void_p), ("wavelet", C.c_void_p),
+                ("src_x", C.c_uint64), ("src_y", C.c_uint64), ("src_z", C.c_uint64)]
+
+
 class stencil_perf(C.Structure):
     _fields_ = [("time_ms", C.c_double), ("bandwidth_gbs", C.c_double), ("gflops", C.c_double),
                 ("model_bandwidth_gbs", C.c_double), ("grid_size", C.c_uint64),
@@ -116,6 +121,12 @@ SIGNATURES = {
     "stencil_forward_step_f64": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _I, _P]),
     "stencil_adjoint_step_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _I, _P]),
     "stencil_adjoint_step_f64": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _I, _P]),
+    "stencil_batch_create": (_I, [C.POINTER(stencil_shot), _I, _I, _I, C.POINTER(_P), _P]),
+    "stencil_batch_run": (_I, [_P, _P]),
+    "stencil_batch_newest": (_I, [_P, _I]),
+    "stencil_batch_groups": (_I, [_P]),
+    "stencil_batch_group": (_I, [_P, _I, _G, C.POINTER(stencil_tile_shape), C.POINTER(_I)]),
+    "stencil_batch_destroy": (None, [_P]),
     "stencil_config_default": (None, [C.POINTER(stencil_config)]),
     "stencil_solver_create": (_I, [C.POINTER(stencil_config), C.POINTER(_P)]),
     "stencil_solver_step": (_I, [_P]),

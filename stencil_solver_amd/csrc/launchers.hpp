@@ -90,6 +90,18 @@ hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, floa
 hipError_t adjoint_zq_f32(const float* lam1, const float* lam2, const float* m, const float* lap, float* grad,
                           float* lam_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
                           float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st);
+// One shot of a batched forward launch (device table entry): three time-level
+// buffers, vel2, the source cell and its wavelet (nullable: no source).
+struct BatchShot {
+    float* ring[3];
+    const float* vel2;
+    const float* wavelet;
+    int src_z, src_y, src_x, pad;
+};
+// The forward FWI step of n_shots shots of equal dims in one launch: shot i
+// steps ring[rot] / ring[rot+1] -> ring[rot+2] (mod 3), adding wavelet[step].
+hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int step, const Dims& d,
+                            const Scalars<float>& s, ZqShape shape, hipStream_t st);
 // Elementwise conversion between FP32 and a 16-bit format (n % 16 == 0).
 hipError_t encode16(int fmt, const float* src, Field16 dst, size_t n, float scale, hipStream_t st);
 hipError_t decode16(int fmt, Field16 src, float* dst, size_t n, float scale, hipStream_t st);

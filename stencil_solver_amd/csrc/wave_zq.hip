@@ -71,6 +71,12 @@ struct ZqArgs {
     const float* src_val;
     float* trace;
     int rec_z, src_z, src_y, src_x;
+    // Batched kWaveFwi: workgroup row blockIdx.y steps shot batch[blockIdx.y]
+    // (all shots of a launch share the dims): p^{n-1}, p^n, p^{n+1} are its
+    // ring[rot], ring[rot+1], ring[rot+2] (mod 3), the source adds
+    // wavelet[step].
+    const BatchShot* batch;
+    int rot, step;
 };
 
 using dev::Fmt;
@@ -178,6 +184,20 @@ void zq_kernel(ZqArgs a) {
     constexpr bool kAdj = (MODE == kAdjoint);
     constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact || kFwd || kAdj);
     static_assert(!(kFwd || kAdj) || FMT == dev::kF32, "FWI modes are FP32");
+    if constexpr (kFwd) {
+        if (a.batch) {  // uniform per workgroup
+            const BatchShot sh = a.batch[blockIdx.y];
+            a.prev = sh.ring[a.rot % 3];
+            a.cur = sh.ring[(a.rot + 1) % 3];
+            a.next = sh.ring[(a.rot + 2) % 3];
+            a.vel2 = sh.vel2;
+            a.src_z = sh.src_z;
+            a.src_y = sh.src_y;
+            a.src_x = sh.src_x;
+            a.src_val = sh.wavelet ? sh.wavelet + a.step : nullptr;
+            a.lap_out = nullptr;
+        }
+    }
     constexpr bool kBlk = (FMT == dev::kBfp16);
     constexpr unsigned kEsz = FMT == dev::kF32 ? 4u : 2u;  // bytes per stored value
     constexpr int kThreads = TXV * TY;
@@ -624,7 +644,7 @@ void fill_weights(ZqArgs& a, double scale) {
 }
 
 hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st,
-                     int fmt = dev::kF32) {
+                     int fmt = dev::kF32, int n_shots = 1) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
     shape = zq_resolve(d, shape, fmt != dev::kF32 ? 2 + fmt : (mode == kHeatFused || mode == kHeatExact ? 1 : 0));
     KernelFn fn = fmt == dev::kF32 ? pick(shape.txv, shape.ty, mode, shape.nt != 0)
@@ -642,7 +662,8 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = (shape.xcd_remap != 0 && nwg % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg)), dim3(shape.txv * shape.ty), 0, st, a);
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg), static_cast<unsigned>(n_shots)), dim3(shape.txv * shape.ty),
+                       0, st, a);
     return hipGetLastError();
 }
 
@@ -755,6 +776,17 @@ hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, floa
     a.s = s;
     shape.nt = 1;
     return launch_zq(a, d, shape, kWaveFwi, st);
+}
+
+hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int step, const Dims& d,
+                            const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+    ZqArgs a{};
+    a.batch = table;
+    a.rot = rot;
+    a.step = step;
+    a.s = s;
+    shape.nt = 1;
+    return launch_zq(a, d, shape, kWaveFwi, st, dev::kF32, n_shots);
 }
 
 hipError_t adjoint_zq_f32(const float* lam1, const float* lam2, const float* m, const float* lap, float* grad,

@@ -51,6 +51,10 @@ def test_forward_is_wave_step_plus_source(oracle):
         assert np.array_equal(tr[k], nxt[rz]), k
         p_prev, p = p, nxt
     assert np.abs(tr).max() > 0
+    # the batched-shot restatement ends on the same field
+    assert np.array_equal(oracle.shot_run(m, np.float32(h), np.float32(dt), n, src, w), p)
+    # and without a source stays zero
+    assert not np.any(oracle.shot_run(m, np.float32(h), np.float32(dt), 5, None, None))
 
 
 def test_gradient_matches_central_differences(oracle):
