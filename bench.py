@@ -190,8 +190,8 @@ def main():
 
     # Measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
     # scaled by a known-size copy), committed under profiles/ for this kernel.
-    tag = args.variant if args.storage == "fp32" else f"{args.variant}_{args.storage}"
-    tfile = ROOT / "profiles" / f"traffic_wave_{tag}_{n}.json"
+    tag = f"wave_{args.variant}_{n}" if args.storage == "fp32" else f"wave16_{args.storage}_{n}"
+    tfile = ROOT / "profiles" / f"traffic_{tag}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
         result["roofline"]["traffic"] = t.get("bytes_per_launch")
@@ -286,7 +286,7 @@ def main():
     if not args.no_fwi:
         result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev)
     if not args.no_batch:
-        result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev)
+        result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev, tune=not args.no_tune)
 
     if args.calibrate:
         # Known-size copy: n^3 floats read + n^3 floats written per launch, the
@@ -405,7 +405,7 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     return out
 
 
-def batch_study(ss, n_shots, steps, dev):
+def batch_study(ss, n_shots, steps, dev, tune=True):
     """Batched multi-shot modelling: n_shots cubic grids with n drawn uniformly
     from {96, 128, 160, 192, 256, 320} (seed 4); each shot: seeded smooth
     velocity (Gaussian-filtered noise, sigma 8 cells) mapped to [1500, 4500]
@@ -428,10 +428,12 @@ def batch_study(ss, n_shots, steps, dev):
                       "src": src})
         del v
     t_inputs = time.time() - t0
+    print(f"[bench] batch inputs: {n_shots} shots in {t_inputs:.1f}s", file=sys.stderr, flush=True)
     t0 = time.time()
-    b = ss.ShotBatch(shots, steps, dev, tune=True)
+    b = ss.ShotBatch(shots, steps, dev, tune=tune)
     torch.cuda.synchronize()
     t_build = time.time() - t0
+    print(f"[bench] batch graph built in {t_build:.1f}s", file=sys.stderr, flush=True)
     b.run()  # warm-up replay
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

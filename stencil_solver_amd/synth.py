@@ -44,20 +44,53 @@ def gaussian_taps(sigma: float, truncate: float = 4.0) -> np.ndarray:
 
 def gaussian_filter(x, sigma: float, truncate: float = 4.0):
     """Separable Gaussian filter of a (nz, ny, nx) torch tensor, edges clamped
-    (scipy.ndimage.gaussian_filter with mode='nearest')."""
+    (scipy.ndimage.gaussian_filter with mode='nearest').  Each axis pass is a
+    sum of shifted slices of the edge-padded field (one multiply-add per tap),
+    so it runs as plain elementwise kernels on any device."""
     import torch
-    import torch.nn.functional as F
 
-    taps = torch.as_tensor(gaussian_taps(sigma, truncate), dtype=x.dtype, device=x.device).view(1, 1, -1)
-    rad = (taps.numel() - 1) // 2
+    taps = gaussian_taps(sigma, truncate)
+    rad = (len(taps) - 1) // 2
     out = x
     for axis in (2, 1, 0):
-        moved = out.movedim(axis, -1)
-        shp = moved.shape
-        flat = moved.reshape(-1, 1, shp[-1])
-        flat = F.conv1d(F.pad(flat, (rad, rad), mode="replicate"), taps)
-        out = flat.reshape(shp).movedim(-1, axis).contiguous()
+        n = out.shape[axis]
+        lo = out.narrow(axis, 0, 1)
+        hi = out.narrow(axis, n - 1, 1)
+        rep = [1, 1, 1]
+        rep[axis] = rad
+        padded = torch.cat([lo.repeat(rep), out, hi.repeat(rep)], dim=axis)
+        acc = torch.zeros_like(out)
+        for k, w in enumerate(taps):
+            acc.add_(padded.narrow(axis, k, n), alpha=float(w))
+        out = acc
     return out
+
+
+def _s64(c: int) -> int:
+    """A uint64 constant as the int64 with the same bits."""
+    return c - (1 << 64) if c >= (1 << 63) else c
+
+
+def _splitmix64_torch(z):
+    """splitmix64 on int64 tensors holding uint64 bit patterns (wrapping
+    multiply; right shifts made logical by masking)."""
+    def srl(v, k):
+        return (v >> k) & ((1 << (64 - k)) - 1)
+
+    z = z + _s64(0x9E3779B97F4A7C15)
+    z = (z ^ srl(z, 30)) * _s64(0xBF58476D1CE4E5B9)
+    z = (z ^ srl(z, 27)) * _s64(0x94D049BB133111EB)
+    return z ^ srl(z, 31)
+
+
+def u01_torch(seed: int, stream: int, n: int, device="cpu"):
+    """u01() computed on `device` (same values)."""
+    import torch
+
+    key = int(splitmix64(np.uint64(seed) ^ splitmix64(np.array([stream], dtype=np.uint64)))[0])
+    idx = torch.arange(n, dtype=torch.int64, device=device) + _s64(key)
+    bits = (_splitmix64_torch(idx) >> 40) & ((1 << 24) - 1)
+    return bits.to(torch.float32) * (2.0 ** -24)
 
 
 def smooth_velocity(shape, sigma: float, vmin: float, vmax: float, seed: int, device="cpu", stream: int = 0):
@@ -66,7 +99,7 @@ def smooth_velocity(shape, sigma: float, vmin: float, vmax: float, seed: int, de
     import torch
 
     nz, ny, nx = shape
-    noise = torch.from_numpy(u01(seed, stream, nz * ny * nx).reshape(shape)).to(device)
+    noise = u01_torch(seed, stream, nz * ny * nx, device).reshape(shape)
     f = gaussian_filter(noise, sigma)
     lo, hi = f.min(), f.max()
     return (vmin + (vmax - vmin) * (f - lo) / (hi - lo)).to(torch.float32).contiguous()

@@ -18,6 +18,11 @@ def test_u01_matches_oracle_generator(oracle):
         assert np.array_equal(got, want)
 
 
+def test_u01_torch_matches_numpy():
+    for seed, stream in ((0, 0), (4, 1), (2 ** 40 + 7, 12345), (2 ** 63 + 5, 2 ** 64 - 1)):
+        assert np.array_equal(synth.u01_torch(seed, stream, 5000).numpy(), synth.u01(seed, stream, 5000))
+
+
 def test_gaussian_filter_matches_scipy():
     import torch
     from scipy import ndimage
