@@ -284,7 +284,7 @@ def main():
                 "speedup_vs_fp32": round(ok[best]["gpts"] / gpts, 3) if best else None}
 
     if not args.no_fwi:
-        result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev)
+        result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev, tune=not args.no_tune)
     if not args.no_batch:
         result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev, tune=not args.no_tune)
 
@@ -458,7 +458,7 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
     return out
 
 
-def fwi_study(ss, n, steps, dev):
+def fwi_study(ss, n, steps, dev, tune=True):
     """Adjoint FWI velocity gradient: n^3 FP32, h = 10 m, dt = 8e-4 s.  True
     model: seeded smooth noise (sigma 10 cells) mapped to [1800, 4000] m/s plus
     a +10 % sphere of radius n/12.8 cells at the centre; starting model without
@@ -477,7 +477,7 @@ def fwi_study(ss, n, steps, dev):
     v_true = torch.where(sphere, v0 * 1.1, v0)
     m0, m_true = (v0 * v0).contiguous(), (v_true * v_true).contiguous()
     w = ss.ricker(steps, dt, 12.0)
-    pr = ss.FwiProblem(grid, steps, (12, n // 2, n // 2), 8, w, dev, torch.float32)
+    pr = ss.FwiProblem(grid, steps, (12, n // 2, n // 2), 8, w, dev, torch.float32, tune=tune)
     d_obs = pr.traces(m_true)
     e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     pr.misfit(m0, d_obs)  # warm-up (first-launch costs)
@@ -500,7 +500,7 @@ def fwi_study(ss, n, steps, dev):
            "misfit": float(J), "grad_l2": float(g.double().norm()),
            "workspace_gb": round(pr.workspace_bytes / 1e9, 3),
            "algorithmic_bytes": alg, "achieved_gbs": round(alg / (grad_ms * 1e-3) / 1e9, 1),
-           "grad_floor_ms_at_8tbs": round(alg / 8e12 * 1e3, 1)}
+           "grad_floor_ms_at_8tbs": round(alg / 8e12 * 1e3, 1), "shape": pr.shape}
     print(f"[bench] fwi {n}^3 x {steps}: grad {grad_ms:.1f} ms (forward {fwd_ms:.1f}, backward {bwd_ms:.1f}), "
           f"{out['achieved_gbs']} GB/s algorithmic", file=sys.stderr, flush=True)
     del pr, d_obs, g, m0, m_true, v0, v_true

@@ -163,11 +163,15 @@ __device__ __forceinline__ void set_comp(float4& v, int i, float f) {
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
-// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Remapping
-// b -> (b % 8) * (n / 8) + b / 8 hands every XCD one contiguous range of
-// logical tiles, so tiles that share halo rows share an L2.  Needs n % 8 == 0.
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  XCD k holds
+// blocks k, k+8, ...: floor(n/8) of them, plus one when k < n % 8.  Mapping
+// block b to k * floor(n/8) + min(k, n % 8) + b / 8 hands every XCD one
+// contiguous range of logical tiles (a bijection of [0, n) for any n), so
+// tiles that share halo rows share an L2.
 __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n, bool on) {
-    return on ? (b & 7u) * (n >> 3) + (b >> 3) : b;
+    if (!on) return b;
+    const unsigned k = b & 7u;
+    return k * (n >> 3) + min(k, n & 7u) + (b >> 3);
 }
 
 }  // namespace stencil::dev

@@ -132,13 +132,17 @@ struct Ops;
 // (bit-identical to the global-memory kernels), else on the latter.
 template <>
 struct Ops<float> {
+    // rec (nullable): plane rec_z of p_next into this nx*ny plane (halo pre-zeroed)
     static hipError_t fwd(const float* p, const float* pp, const float* m, float* pn, float* lap, const StepCtx& c,
-                          const Scalars<float>& s, const float* w, int n, hipStream_t st) {
+                          const Scalars<float>& s, const float* w, int n, float* rec, int rec_z, hipStream_t st) {
         const void* ptrs[4] = {p, pp, m, pn};
-        if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || aligned16(lap)))
-            return wave_zq_fwi_f32(p, pp, m, pn, lap, c.src_z, c.src_y, c.src_x, c.src >= 0 ? w + n : nullptr, c.d,
-                                   s, c.shape, st);
-        return fwi_fwd_step_f32(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
+        if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || aligned16(lap)) && (!rec || aligned16(rec)))
+            return wave_zq_fwi_f32(p, pp, m, pn, lap, c.src_z, c.src_y, c.src_x, c.src >= 0 ? w + n : nullptr, rec,
+                                   rec_z, c.d, s, c.shape, st);
+        const hipError_t e = fwi_fwd_step_f32(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
+        if (e != hipSuccess || !rec) return e;
+        const size_t plane = size_t(c.d.nx) * size_t(c.d.ny);
+        return hipMemcpyAsync(rec, pn + size_t(rec_z) * plane, plane * sizeof(float), hipMemcpyDeviceToDevice, st);
     }
     static hipError_t adj(const float* l1, const float* l2, const float* m, const float* lap, float* g, float* lo,
                           const float* res, int rz, const StepCtx& c, const Scalars<float>& s, float* tr,
@@ -159,8 +163,12 @@ struct Ops<float> {
 template <>
 struct Ops<double> {
     static hipError_t fwd(const double* p, const double* pp, const double* m, double* pn, double* lap,
-                          const StepCtx& c, const Scalars<double>& s, const double* w, int n, hipStream_t st) {
-        return fwi_fwd_step_f64(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
+                          const StepCtx& c, const Scalars<double>& s, const double* w, int n, double* rec, int rec_z,
+                          hipStream_t st) {
+        const hipError_t e = fwi_fwd_step_f64(p, pp, m, pn, lap, c.d, s, c.src, w, n, st);
+        if (e != hipSuccess || !rec) return e;
+        const size_t plane = size_t(c.d.nx) * size_t(c.d.ny);
+        return hipMemcpyAsync(rec, pn + size_t(rec_z) * plane, plane * sizeof(double), hipMemcpyDeviceToDevice, st);
     }
     static hipError_t adj(const double* l1, const double* l2, const double* m, const double* lap, double* g,
                           double* lo, const double* res, int rz, const StepCtx& c, const Scalars<double>& s,
@@ -205,11 +213,11 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
     FWI_TRY(Ops<T>::fill(ring[0], p.cells, st));
     FWI_TRY(Ops<T>::fill(ring[1], p.cells, st));
     FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
+    FWI_TRY(Ops<T>::fill(traces, size_t(p.n) * p.plane, st));  // the steps write the plane interiors
     int prev = 0, cur = 1, next = 2;
     for (int64_t n = 0; n < p.n; ++n) {
-        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, cx, s, wavelet, int(n), st));
-        FWI_TRY(hipMemcpyAsync(traces + size_t(n) * p.plane, ring[next] + size_t(p.rec_z) * p.plane,
-                               p.plane * sizeof(T), hipMemcpyDeviceToDevice, st));
+        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, cx, s, wavelet, int(n),
+                            traces + size_t(n) * p.plane, p.rec_z, st));
         if (n % p.C == 0) {
             T* ck = at<T>(work, p.ckpt + size_t(2 * (n / p.C)) * field);
             FWI_TRY(hipMemcpyAsync(ck, ring[cur], p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -263,7 +271,7 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
         int prev = 0, cur = 1, next = 2;
         for (int64_t j = k_lo; j <= std::min(k_hi, p.n - 1); ++j) {
             T* lap = at<T>(work, p.lap + size_t(j - k_lo) * field);
-            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(j), st));
+            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(j), nullptr, 0, st));
             const int t = prev;
             prev = cur;
             cur = next;
@@ -340,7 +348,8 @@ int fwd_step_api(const T* p, const T* pp, const T* m, T* pn, T* lap_out, const s
     if (src >= cells || (src >= 0 && (!wavelet || n < 0))) return STENCIL_ERR_INVALID;
     StepCtx cx = step_ctx(dims_of_grid(g), src >= 0 ? (long long)src : -1LL);
     cx.tiled = variant != STENCIL_VARIANT_NAIVE;
-    return Ops<T>::fwd(p, pp, m, pn, lap_out, cx, scalars<T>(g), wavelet, int(n), static_cast<hipStream_t>(stream)) ==
+    return Ops<T>::fwd(p, pp, m, pn, lap_out, cx, scalars<T>(g), wavelet, int(n), nullptr, 0,
+                       static_cast<hipStream_t>(stream)) ==
                    hipSuccess
                ? STENCIL_OK
                : STENCIL_ERR_HIP;

@@ -90,8 +90,7 @@ std::vector<ZqShape> candidates(const Dims& d) {
         if (c == 1 || inz / c >= 16) chunks.push_back(c);
     std::vector<ZqShape> out;
     // xcd_remap 0 / 1: workgroup order as dispatched, or each XCD a
-    // contiguous range of tiles (1 needs a workgroup count divisible by 8;
-    // the launcher falls back to 0 otherwise, so such rows repeat 0).
+    // contiguous range of tiles.
     for (const auto& sh : shapes)
         for (int c : chunks)
             for (int nt : {1, 0})

@@ -59,7 +59,8 @@ struct ZqArgs {
     float w0, wx[4], wy[4], wz[4];   // fused weights (heat: pre-multiplied by coef)
     // FWI modes (FP32).  kWaveFwi: cur = p^n, prev = p^{n-1}, vel2 = m,
     // next = p^{n+1}, lap_out = L p^n (nullable), src_val = &wavelet[n]
-    // (nullable) added at cell (src_z, src_y, src_x).  kAdjoint: cur =
+    // (nullable) added at cell (src_z, src_y, src_x), trace (nullable) = the
+    // nx*ny receiver plane that receives p^{n+1} on plane rec_z.  kAdjoint: cur =
     // lambda^{k+1}, aux = m, prev = lambda^{k+2}, vel2 = L p^k (nullable: no
     // gradient term), grad updated in place, next = lambda^k, res = residual
     // plane (nullable) added on plane rec_z, trace (nullable) = lambda^k at
@@ -186,15 +187,18 @@ void zq_kernel(ZqArgs a) {
     static_assert(!(kFwd || kAdj) || FMT == dev::kF32, "FWI modes are FP32");
     if constexpr (kFwd) {
         if (a.batch) {  // uniform per workgroup
-            const BatchShot sh = a.batch[blockIdx.y];
-            a.prev = sh.ring[a.rot % 3];
-            a.cur = sh.ring[(a.rot + 1) % 3];
-            a.next = sh.ring[(a.rot + 2) % 3];
-            a.vel2 = sh.vel2;
-            a.src_z = sh.src_z;
-            a.src_y = sh.src_y;
-            a.src_x = sh.src_x;
-            a.src_val = sh.wavelet ? sh.wavelet + a.step : nullptr;
+            // Fields read straight from the table (a private copy of the entry
+            // indexed by rot would live in scratch memory).
+            const BatchShot* sh = a.batch + blockIdx.y;
+            const int r = a.rot % 3;
+            a.prev = sh->ring[r];
+            a.cur = sh->ring[r == 2 ? 0 : r + 1];
+            a.next = sh->ring[r == 0 ? 2 : r - 1];
+            a.vel2 = sh->vel2;
+            a.src_z = sh->src_z;
+            a.src_y = sh->src_y;
+            a.src_x = sh->src_x;
+            a.src_val = sh->wavelet ? sh->wavelet + a.step : nullptr;
             a.lap_out = nullptr;
         }
     }
@@ -291,6 +295,7 @@ void zq_kernel(ZqArgs a) {
         return load_raw<FMT, NT>(plane_rsrc(f, z, plane_bytes), own_ld);
     };
     auto exp_at = [eplane](const int8_t* e, int z, unsigned eoff) -> int {
+        (void)eplane;
         if constexpr (kBlk)
             return static_cast<signed char>(
                 __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(e, z, eplane), static_cast<int>(eoff), 0, 0));
@@ -377,6 +382,8 @@ void zq_kernel(ZqArgs a) {
                 const bool at_src = own_out && zz == a.src_z && y == a.src_y && a.src_x >= x && a.src_x < x + 4;
                 if constexpr (kFwd) {
                     if (at_src && a.src_val) dev::set_comp(out, a.src_x - x, comp(out, a.src_x - x) + *a.src_val);
+                    // receiver plane: p^{n+1} on plane rec_z into the trace (its halo is pre-zeroed)
+                    if (a.trace && zz == a.rec_z && own_out) *reinterpret_cast<float4*>(a.trace + own_off) = out;
                     if (a.lap_out)
                         store4<FMT, NT, true>(plane_rsrc(a.lap_out, zz, plane_bytes), plane_rsrc(a.lap_out, zz, 0),
                                               own_b, 0, lap4, 1.f, own_out, false);
@@ -661,7 +668,7 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     a.chunk = static_cast<int>(((inz + shape.n_chunks - 1) / shape.n_chunks + 8) / 9 * 9);
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
-    a.xcd_remap = (shape.xcd_remap != 0 && nwg % 8 == 0) ? 1 : 0;
+    a.xcd_remap = shape.xcd_remap != 0 ? 1 : 0;
     hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg), static_cast<unsigned>(n_shots)), dim3(shape.txv * shape.ty),
                        0, st, a);
     return hipGetLastError();
@@ -761,9 +768,11 @@ hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Fi
 }
 
 hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, float* pn, float* lap_out,
-                           int src_z, int src_y, int src_x, const float* src_val, const Dims& d,
-                           const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+                           int src_z, int src_y, int src_x, const float* src_val, float* rec_plane, int rec_z,
+                           const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st) {
     ZqArgs a{};
+    a.trace = rec_plane;
+    a.rec_z = rec_z;
     a.cur = p;
     a.prev = pp;
     a.vel2 = m;

@@ -39,10 +39,11 @@ class FwiProblem:
     with its device workspace.  ``dtype`` is torch.float32 or torch.float64;
     ``variant`` picks the FP32 step kernels ("tiled": the LDS z-queue kernel,
     "naive": the global-memory one; both give bit-identical results, FP64
-    always uses the latter)."""
+    always uses the latter).  ``tune`` runs the launch-shape autotuner on
+    this grid first (FP32 tiled only); the steps then use the tuned shape."""
 
     def __init__(self, grid: GridDims, n_steps: int, src: tuple[int, int, int], rec_z: int, wavelet,
-                 device, dtype=None, ckpt_every: int = 0, variant: str = "tiled"):
+                 device, dtype=None, ckpt_every: int = 0, variant: str = "tiled", tune: bool = False):
         import torch
 
         self.torch = torch
@@ -76,6 +77,23 @@ class FwiProblem:
         self.misfit_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._have_residual = False
         self._sfx = "f32" if self.dtype == torch.float32 else "f64"
+        self.shape = None
+        if tune and self.dtype == torch.float32 and variant == "tiled":
+            self.shape = self.tune()
+
+    def tune(self, reps: int = 3) -> dict:
+        """Autotune the tiled step for this grid on scratch fields (a smooth
+        pulse in a constant model); returns the shape the steps will use."""
+        from .api import tune_tiled
+
+        torch = self.torch
+        shape = (self.grid.nz, self.grid.ny, self.grid.nx)
+        f = [torch.zeros(shape, dtype=torch.float32, device=self.device) for _ in range(4)]
+        f[2].fill_(4.0e6)
+        f[0][4:-4, 4:-4, 4:-4] = 1.0
+        best = tune_tiled(f[0], f[1], f[2], f[3], self.grid, reps=reps)
+        del f
+        return best
 
     @property
     def ckpt_every(self) -> int:

@@ -107,6 +107,25 @@ def test_tiled_fwi_equals_naive(ss, dev, shape, tuned):
     ss.clear_tuning()
 
 
+def test_fwi_problem_tune(ss, dev):
+    """FwiProblem(tune=True) tunes its grid; its traces stay bit-identical to
+    the global-memory kernels' (receiver plane written by the tiled step)."""
+    import torch
+
+    shape, n = (36, 40, 104), 24
+    m = smooth_model(shape, 3, 1800.0, 3000.0, np.float32)
+    w = (1e3 * ss.ricker(n, 1e-3, 20.0)).astype(np.float32)
+    grid = _grid(ss, shape, 10.0, 1e-3, np.float32)
+    md = torch.from_numpy(m).to(dev)
+    ss.clear_tuning()
+    pt = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, tune=True)
+    assert pt.shape is not None and ss.tuned_shape(grid)[1]
+    pn = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, variant="naive")
+    a, b = pt.traces(md), pn.traces(md)
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()) and a.abs().max() > 0
+    ss.clear_tuning()
+
+
 def test_gradient_matches_finite_differences_fp64(ss, dev):
     """Central differences of J on 16 random interior voxels (FP64), 40^3 x 160
     steps with a velocity anomaly between source and receivers; agreement
