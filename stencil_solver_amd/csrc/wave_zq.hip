@@ -21,8 +21,10 @@
 //     non-temporal accesses so they do not displace p_cur lines in L2.
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
 // write p_next once = 16 B (halo re-reads are L2 hits; the tile order keeps
-// neighbouring tiles on one XCD).
+// neighbouring tiles on one XCD).  Chained steps alternate the z direction
+// (ZqArgs::mirror).
 #include <type_traits>
+#include <cstdlib>
 
 #include "device_common.hpp"
 #include "launchers.hpp"
@@ -78,6 +80,12 @@ struct ZqArgs {
     // wavelet[step].
     const BatchShot* batch;
     int rot, step;
+    // Sweep direction: 1 = the workgroups walk z downwards (the kernel's plane
+    // z is memory plane nz-1-z; src_z / rec_z arrive already mirrored).  The
+    // step is symmetric in z, so both directions give bit-identical results;
+    // alternating them between chained steps starts each step on the planes
+    // the previous one touched last, which are still in the memory-side cache.
+    int mirror;
 };
 
 using dev::Fmt;
@@ -200,6 +208,7 @@ void zq_kernel(ZqArgs a) {
             a.src_x = sh->src_x;
             a.src_val = sh->wavelet ? sh->wavelet + a.step : nullptr;
             a.lap_out = nullptr;
+            if (a.mirror) a.src_z = a.nz - 1 - a.src_z;
         }
     }
     constexpr bool kBlk = (FMT == dev::kBfp16);
@@ -288,17 +297,24 @@ void zq_kernel(ZqArgs a) {
         h_eld[k] = h_in[k] ? (h_off[k] >> 4) : 0u;
     }
     const int z_last = a.nz - 1;
-    auto raw_at = [plane_bytes](const char* f, int z, unsigned boff) -> Raw<FMT> {
-        return load_raw<FMT, false>(plane_rsrc(f, z, plane_bytes), boff);
+    // Kernel plane z -> memory plane (uniform: scalar arithmetic).
+    const int z_sgn = a.mirror ? -1 : 1;
+    const int z_org = a.mirror ? z_last : 0;
+    auto prs = [z_sgn, z_org](const void* base, int z, size_t bytes) {
+        return plane_rsrc(base, z_org + z_sgn * z, bytes);
     };
-    auto raw_stream = [plane_bytes, own_ld](const char* f, int z) -> Raw<FMT> {
-        return load_raw<FMT, NT>(plane_rsrc(f, z, plane_bytes), own_ld);
+    auto raw_at = [plane_bytes, prs](const char* f, int z, unsigned boff) -> Raw<FMT> {
+        return load_raw<FMT, false>(prs(f, z, plane_bytes), boff);
     };
-    auto exp_at = [eplane](const int8_t* e, int z, unsigned eoff) -> int {
+    auto raw_stream = [plane_bytes, own_ld, prs](const char* f, int z) -> Raw<FMT> {
+        return load_raw<FMT, NT>(prs(f, z, plane_bytes), own_ld);
+    };
+    auto exp_at = [eplane, prs](const int8_t* e, int z, unsigned eoff) -> int {
         (void)eplane;
+        (void)prs;
         if constexpr (kBlk)
             return static_cast<signed char>(
-                __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(e, z, eplane), static_cast<int>(eoff), 0, 0));
+                __builtin_amdgcn_raw_buffer_load_b8(prs(e, z, eplane), static_cast<int>(eoff), 0, 0));
         return 0;
     };
     // This lane's quad (16 x-consecutive cells) holds output cells: it owns
@@ -385,7 +401,7 @@ void zq_kernel(ZqArgs a) {
                     // receiver plane: p^{n+1} on plane rec_z into the trace (its halo is pre-zeroed)
                     if (a.trace && zz == a.rec_z && own_out) *reinterpret_cast<float4*>(a.trace + own_off) = out;
                     if (a.lap_out)
-                        store4<FMT, NT, true>(plane_rsrc(a.lap_out, zz, plane_bytes), plane_rsrc(a.lap_out, zz, 0),
+                        store4<FMT, NT, true>(prs(a.lap_out, zz, plane_bytes), prs(a.lap_out, zz, 0),
                                               own_b, 0, lap4, 1.f, own_out, false);
                 } else {
                     if (a.res && zz == a.rec_z && own_out) {
@@ -395,14 +411,14 @@ void zq_kernel(ZqArgs a) {
                     if (a.vel2) {  // gradient: grad += dt^2 * (L p^k) * lambda^{k+1}
                         const float4 g = make_float4(gr.x + a.s.dt2 * vv.x * lc.x, gr.y + a.s.dt2 * vv.y * lc.y,
                                                      gr.z + a.s.dt2 * vv.z * lc.z, gr.w + a.s.dt2 * vv.w * lc.w);
-                        store4<FMT, NT, true>(plane_rsrc(a.grad, zz, plane_bytes), plane_rsrc(a.grad, zz, 0),
+                        store4<FMT, NT, true>(prs(a.grad, zz, plane_bytes), prs(a.grad, zz, 0),
                                               own_b, 0, g, 1.f, own_out, false);
                     }
                     if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x);
                 }
             }
-            store4<FMT, NT, kExact>(plane_rsrc(next_b, zz, plane_bytes),
-                                    plane_rsrc(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
+            store4<FMT, NT, kExact>(prs(next_b, zz, plane_bytes),
+                                    prs(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
                                     a.p_scale, own_out, blk_out);
         }
     };
@@ -650,6 +666,37 @@ void fill_weights(ZqArgs& a, double scale) {
     }
 }
 
+// Sweep direction of a launch: the opposite of the launch that wrote this
+// one's input field (a chained time step), else upwards.  Remembers the last 8
+// launches of the calling thread; STENCIL_ALTERNATE_SWEEP=0 keeps every launch
+// upwards.  Direction never changes results, only which planes come first.
+bool sweep_alternation() {
+    static const bool enabled = [] {
+        const char* e = std::getenv("STENCIL_ALTERNATE_SWEEP");
+        return !(e && e[0] == '0');
+    }();
+    return enabled;
+}
+
+int sweep_direction(const void* cur, const void* next) {
+    struct Recent {
+        const void* next;
+        int mirror;
+    };
+    static thread_local Recent recent[8] = {};
+    static thread_local int head = 0;
+    if (!sweep_alternation() || !cur) return 0;
+    int m = 0;
+    for (const Recent& r : recent)
+        if (r.next == cur) {
+            m = 1 - r.mirror;
+            break;
+        }
+    recent[head] = Recent{next, m};
+    head = (head + 1) & 7;
+    return m;
+}
+
 hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream_t st,
                      int fmt = dev::kF32, int n_shots = 1) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
@@ -669,6 +716,15 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = shape.xcd_remap != 0 ? 1 : 0;
+    if (a.batch) {
+        a.mirror = sweep_alternation() ? (a.step & 1) : 0;  // one shot's nodes chain step after step
+    } else {
+        a.mirror = sweep_direction(a.cur, a.next);
+        if (a.mirror) {
+            a.src_z = a.nz - 1 - a.src_z;
+            a.rec_z = a.nz - 1 - a.rec_z;
+        }
+    }
     hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(nwg), static_cast<unsigned>(n_shots)), dim3(shape.txv * shape.ty),
                        0, st, a);
     return hipGetLastError();

@@ -184,6 +184,30 @@ def test_tiled_default_vs_oracle(ss, dev, oracle, shape):
     assert np.array_equal(ring_run(ss, "tiled", p0, vel2, grid, 8, dev), ref)
 
 
+@pytest.mark.parametrize("exact", [True, False])
+def test_alternating_sweep_bit_identical(ss, dev, oracle, exact):
+    """Chained steps sweep z alternately down and up (each step starts on the
+    planes the previous one wrote last); steps whose input is a fresh copy
+    always sweep upwards.  Both give the same bits."""
+    import torch
+
+    shape = (44, 36, 72)
+    p0, vel2 = _inputs(oracle, shape, 21)
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
+                       np.float32(8e-4))
+    kw = dict(tx4=32, ty=8, n_chunks=3, exact=exact)
+    chained = ring_run(ss, "tiled_ex", p0, vel2, grid, 9, dev, **kw)
+    prev, cur, v = to_dev(p0, dev), to_dev(p0, dev), to_dev(vel2, dev)
+    for _ in range(9):
+        nxt = torch.zeros_like(cur)
+        ss.launch_tiled_ex(cur.clone(), prev.clone(), v, nxt, grid, **kw)
+        prev, cur = cur, nxt
+    torch.cuda.synchronize()
+    assert np.array_equal(chained, cur.cpu().numpy())
+    if exact:
+        assert np.array_equal(chained, oracle.wave_run(p0, vel2, grid.dx, grid.dt, 9, grid.dy, grid.dz))
+
+
 def test_layered_model_128_50_steps(ss, dev, oracle):
     """The 512^3 workload's velocity model at 128^3: 16 layers, +-5 % noise."""
     shape = (128, 128, 128)
