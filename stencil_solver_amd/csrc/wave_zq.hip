@@ -15,9 +15,10 @@
 //     float4, written from the queue centre plus the halo float4 (top/bottom 4
 //     rows, left/right 4 columns; corners are never read by the star stencil).
 //     LDS is double-buffered, so one barrier per plane suffices.
-//   * p_prev, vel2 and p_next are touched once per step; their loads and the
-//     halo loads for plane z+1 are issued before plane z is computed (gfx950
-//     barriers do not drain outstanding global loads).  With NT they use
+//   * p_prev, vel2 and p_next are touched once per step.  Every load is issued
+//     one step ahead: the queue refill for plane z+5 and the stream and halo
+//     loads for plane z+1 go out before plane z is computed (gfx950 barriers
+//     do not drain outstanding global loads).  With NT they use
 //     non-temporal accesses so they do not displace p_cur lines in L2.
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
 // write p_next once = 16 B (halo re-reads are L2 hits; the tile order keeps
@@ -399,7 +400,9 @@ void zq_kernel(ZqArgs a) {
                 if constexpr (kFwd) {
                     if (at_src && a.src_val) dev::set_comp(out, a.src_x - x, comp(out, a.src_x - x) + *a.src_val);
                     // receiver plane: p^{n+1} on plane rec_z into the trace (its halo is pre-zeroed)
-                    if (a.trace && zz == a.rec_z && own_out) *reinterpret_cast<float4*>(a.trace + own_off) = out;
+                    if (a.trace && zz == a.rec_z)
+                        store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), plane_rsrc(a.trace, 0, 0),
+                                                 own_b, 0, out, 1.f, own_out, false);
                     if (a.lap_out)
                         store4<FMT, NT, true>(prs(a.lap_out, zz, plane_bytes), prs(a.lap_out, zz, 0),
                                               own_b, 0, lap4, 1.f, own_out, false);
@@ -450,12 +453,15 @@ void zq_kernel(ZqArgs a) {
     }
 
     if constexpr (FMT == dev::kF32) {
-        // FP32: the queue refill (plane zz+4) is loaded at the top of step zz;
-        // halo and streams for plane zz+1 are prefetched one step ahead.
-        // (kAdjoint also prefetches m for the halo, lambda1 at the point, and
-        // the gradient; its vel2 stream is L p^k, absent when vel2 is null.)
+        // FP32: every load is issued one step ahead: the queue refill for
+        // plane zz+5 (kAdjoint: lambda and m, multiplied when they enter the
+        // queue), halo and streams for plane zz+1.  (kAdjoint also prefetches
+        // m for the halo, lambda1 at the point, and the gradient; its vel2
+        // stream is L p^k, absent when vel2 is null.)
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const bool lap_in = kAdj && a.vel2 != nullptr;
+        float4 rq = raw_at(cur_b, min(z_begin + kR, z_last), own_ld), rqm = zero4;
+        if constexpr (kAdj) rqm = raw_at(aux_b, min(z_begin + kR, z_last), own_ld);
         float4 h_cur[kHaloPer], hm_cur[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
@@ -476,7 +482,13 @@ void zq_kernel(ZqArgs a) {
             for (int j = 0; j < 9; ++j) {
                 const int zz = zb + j;
                 const int z1 = min(zz + 1, z_last);
-                q[(j + 8) % 9] = staged(min(zz + kR, z_last), own_ld);
+                const int z5 = min(zz + kR + 1, z_last);
+                if constexpr (kAdj)
+                    q[(j + 8) % 9] = make_float4(rqm.x * rq.x, rqm.y * rq.y, rqm.z * rq.z, rqm.w * rq.w);
+                else
+                    q[(j + 8) % 9] = rq;
+                rq = raw_at(cur_b, z5, own_ld);
+                if constexpr (kAdj) rqm = raw_at(aux_b, z5, own_ld);
                 float4 h_nxt[kHaloPer], hm_nxt[kHaloPer];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k) {
