@@ -211,6 +211,8 @@ def main():
         # Heat step (dedicated single-level kernel): 8 B / point.
         a, b = p0.clone(), torch.zeros_like(p0)
         hgrid = ss.GridDims.cube(n, h, np.float32(np.sqrt(np.float64(5.0))))
+        if not args.no_tune:
+            extras["heat_shape"] = ss.tune_heat(a, b, hgrid, reps=3)
         for _ in range(5):
             ss.heat_step(a, b, 1.0, hgrid)
             a, b = b, a

@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -168,6 +170,33 @@ __device__ __forceinline__ void set_comp(float4& v, int i, float f) {
 // block b to k * floor(n/8) + min(k, n % 8) + b / 8 hands every XCD one
 // contiguous range of logical tiles (a bijection of [0, n) for any n), so
 // tiles that share halo rows share an L2.
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1, gfx9): lane i gets
+// lane i-1's (shr) or lane i+1's (shl) value of v; the lane with no source
+// (0 for shr, 63 for shl) gets `edge`.
+__device__ __forceinline__ float4 wave_shr1(float4 v, float4 edge) {
+    auto s = [](float e, float x) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(x), 0x138, 0xF, 0xF, false));
+    };
+    return make_float4(s(edge.x, v.x), s(edge.y, v.y), s(edge.z, v.z), s(edge.w, v.w));
+}
+__device__ __forceinline__ float4 wave_shl1(float4 v, float4 edge) {
+    auto s = [](float e, float x) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(x), 0x130, 0xF, 0xF, false));
+    };
+    return make_float4(s(edge.x, v.x), s(edge.y, v.y), s(edge.z, v.z), s(edge.w, v.w));
+}
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
+// a loop whose index is a compile-time constant in every iteration.
+template <typename F, int... J>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n, bool on) {
     if (!on) return b;
     const unsigned k = b & 7u;
