@@ -855,18 +855,20 @@ ZqShape zq_resolve(const Dims& d, ZqShape s, int kind) {
                 best_waste = w;
             }
         }
+        if (kind == 1 && best == 64) best = 32;  // heat: 128 x 16 cells (same padding)
         s.txv = best;
         s.ty = best == 64 ? ((kind >= 2 || d.nx >= 1024) ? 16 : 8) : best == 32 ? 16 : 32;
     }
     if (s.nt < 0) s.nt = 1;
     const int64_t inz = d.nz - 2 * kR;
     if (s.n_chunks <= 0) {
-        // z-chunks for about one workgroup per CU (256 CUs; two for the
-        // single-level heat kernel), chunks of at least 32 planes: long
-        // z-marches stream best.  These are the autotuner's picks at 512^3.
+        // z-chunks for about one workgroup per CU (256 CUs), chunks of at
+        // least 32 planes: long z-marches stream best.  These are the
+        // autotuner's picks at 512^3 (wave 256 x 8 cells, heat 128 x 16,
+        // both 2 chunks).
         const int64_t tiles =
             ((d.nx + 4 * s.txv - 1) / (4 * s.txv)) * ((d.ny - 2 * kR + s.ty - 1) / s.ty);
-        const int64_t target = kind == 1 ? 512 : 256;
+        const int64_t target = 256;
         int64_t want = (target + tiles / 2) / tiles;
         const int64_t max_chunks = inz >= 32 ? inz / 32 : 1;
         if (want > max_chunks) want = max_chunks;

@@ -348,6 +348,38 @@ def test_fullsize_tiled_equals_naive(ss, dev, n):
     assert torch.equal(a, torch.flip(a, dims=[2]))  # constant-in-x medium, centred pulse
 
 
+def test_fullsize_heat_tiled_equals_naive(ss, dev):
+    """The 512^3 heat workload (U[0,1] noise + a hot cube of side 64, r = 0.05):
+    tiled (exact association, default and tuned shape, chained steps sweeping
+    z both ways) == naive bit for bit over 4 steps; the fused form within 1e-6
+    rel-L2.  (No maximum principle: the 8th-order weights c2, c4 are negative,
+    so the cube's edges overshoot by a few per cent.)"""
+    import torch
+
+    n = 512
+    shape = (n, n, n)
+    t0 = to_dev(ss.host_noise_cube(shape, 64, 10.0, 2), dev)
+    grid = ss.GridDims.cube(n, np.float32(10.0), np.float32(np.sqrt(np.float64(5.0))))
+
+    def run(steps, **kw):
+        a, b = t0.clone(), torch.zeros_like(t0)
+        for _ in range(steps):
+            ss.heat_step(a, b, 1.0, grid, **kw)
+            a, b = b, a
+        torch.cuda.synchronize()
+        return a
+
+    ref = run(4, variant="naive")
+    assert torch.equal(run(4, exact=True), ref)
+    ss.tune_heat(t0.clone(), torch.zeros_like(t0), grid, reps=1)
+    assert torch.equal(run(4, exact=True), ref)
+    fused = run(4, exact=False)
+    rel = torch.linalg.vector_norm((fused - ref).double()) / torch.linalg.vector_norm(ref.double())
+    assert rel.item() <= 1e-6
+    assert not torch.equal(ref, t0) and 10.0 < ref.max().item() < 10.5  # heat spread, bounded overshoot
+    ss.clear_tuning()
+
+
 # ── autotuner ─────────────────────────────────────────────────────────────────
 def test_autotuner_picks_and_caches_shape(ss, dev, oracle, tmp_path):
     import torch
