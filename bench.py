@@ -505,7 +505,32 @@ def fwi_study(ss, n, steps, dev, tune=True):
            "grad_floor_ms_at_8tbs": round(alg / 8e12 * 1e3, 1), "shape": pr.shape}
     print(f"[bench] fwi {n}^3 x {steps}: grad {grad_ms:.1f} ms (forward {fwd_ms:.1f}, backward {bwd_ms:.1f}), "
           f"{out['achieved_gbs']} GB/s algorithmic", file=sys.stderr, flush=True)
-    del pr, d_obs, g, m0, m_true, v0, v_true
+    del pr
+    torch.cuda.empty_cache()
+    # One segment (ckpt_every = steps): every L p^k stays in HBM, so the
+    # backward pass is the adjoint steps alone (48 B/pt/step instead of 64).
+    cfg_all = dict(ckpt_every=steps)
+    need = 1.05 * 4.0 * n ** 3 * (steps + 10)
+    if torch.cuda.mem_get_info()[0] > need:
+        pa = ss.FwiProblem(grid, steps, (12, n // 2, n // 2), 8, w, dev, torch.float32, tune=False, **cfg_all)
+        pa.misfit(m0, d_obs)
+        pa.gradient(m0)
+        torch.cuda.synchronize()
+        e[0].record()
+        pa.misfit(m0, d_obs)
+        e[1].record()
+        ga = pa.gradient(m0)
+        e[2].record()
+        torch.cuda.synchronize()
+        fa, ba = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+        out["store_all"] = {"grad_ms": round(fa + ba, 2), "forward_ms": round(fa, 2), "backward_ms": round(ba, 2),
+                            "workspace_gb": round(pa.workspace_bytes / 1e9, 2),
+                            "grad_bit_identical": bool(torch.equal(ga, g))}
+        print(f"[bench] fwi store-all L p ({out['store_all']['workspace_gb']} GB): grad {fa + ba:.1f} ms "
+              f"(forward {fa:.1f}, backward {ba:.1f}), identical {out['store_all']['grad_bit_identical']}",
+              file=sys.stderr, flush=True)
+        del pa, ga
+    del d_obs, g, m0, m_true, v0, v_true
     torch.cuda.empty_cache()
     return out
 

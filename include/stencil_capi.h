@@ -241,14 +241,17 @@ int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t 
  * lambda^{k+1}, lambda^k = 2lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1})
  * + residual^k on plane rec_z.  The forward pass keeps the level pair at
  * every ckpt_every-th step; the backward pass recomputes each segment from
- * it.  All device memory is the caller's workspace.  Arrays: vel2 and grad
+ * it.  With ckpt_every = n_steps (one segment: n_steps fields of workspace,
+ * e.g. 134 GB for 256^3 x 2000 FP32) the misfit forward stores every L p^k
+ * and the backward runs the adjoint steps only, with the same results.
+ * All device memory is the caller's workspace.  Arrays: vel2 and grad
  * nx*ny*nz, wavelet n_steps (device), traces / d_obs n_steps*ny*nx with trace
  * n holding p^{n+1} on plane rec_z. ---- */
 typedef struct stencil_fwi_config {
     uint64_t n_steps;
     uint64_t src_x, src_y, src_z; /* source cell (interior) */
     uint64_t rec_z;               /* receiver plane (interior z), all nx*ny cells */
-    uint64_t ckpt_every;          /* segment length; 0 = ceil(sqrt(n_steps)) */
+    uint64_t ckpt_every;          /* segment length; 0 = ceil(sqrt(n_steps)); n_steps = no recompute */
     uint64_t variant;             /* stencil_variant of the FP32 steps (FP64: naive kernels) */
 } stencil_fwi_config;
 /* Workspace bytes (256-byte aligned base) for this grid, config and value

@@ -214,9 +214,14 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
     FWI_TRY(Ops<T>::fill(ring[1], p.cells, st));
     FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
     FWI_TRY(Ops<T>::fill(traces, size_t(p.n) * p.plane, st));  // the steps write the plane interiors
+    // One segment (ckpt_every = n_steps): the workspace holds L p^k of every
+    // step, so a misfit forward writes them now and the backward needs no
+    // recompute (lap slot k-1 holds L p^k, as the segment recompute fills it).
+    const bool keep_lap = p.C == p.n && d_obs != nullptr;
     int prev = 0, cur = 1, next = 2;
     for (int64_t n = 0; n < p.n; ++n) {
-        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], nullptr, cx, s, wavelet, int(n),
+        T* lap = keep_lap && n >= 1 ? at<T>(work, p.lap + size_t(n - 1) * field) : nullptr;
+        FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(n),
                             traces + size_t(n) * p.plane, p.rec_z, st));
         if (n % p.C == 0) {
             T* ck = at<T>(work, p.ckpt + size_t(2 * (n / p.C)) * field);
@@ -257,19 +262,22 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     FWI_TRY(Ops<T>::fill(grad, p.cells, st));
     for (auto* l : lam) FWI_TRY(Ops<T>::fill(l, p.cells, st));
     int l1 = 0, l2 = 1, lo = 2;  // lambda^{k+1}, lambda^{k+2}, lambda^k
+    const bool lap_stored = p.C == p.n;  // the misfit forward wrote every L p^k
     for (int64_t sg = p.S - 1; sg >= 0; --sg) {
         const int64_t k_lo = sg * p.C + 1;
         const int64_t k_hi = std::min((sg + 1) * p.C, p.n);
         // re-run the forward steps of this segment from its checkpoint
-        const T* ck = at<T>(work, p.ckpt + size_t(2 * sg) * field);
-        FWI_TRY(hipMemcpyAsync(ring[0], ck, p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
-        FWI_TRY(hipMemcpyAsync(ring[1], reinterpret_cast<const char*>(ck) + field, p.cells * sizeof(T),
-                               hipMemcpyDeviceToDevice, st));
+        if (!lap_stored) {
+            const T* ck = at<T>(work, p.ckpt + size_t(2 * sg) * field);
+            FWI_TRY(hipMemcpyAsync(ring[0], ck, p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
+            FWI_TRY(hipMemcpyAsync(ring[1], reinterpret_cast<const char*>(ck) + field, p.cells * sizeof(T),
+                                   hipMemcpyDeviceToDevice, st));
+        }
         // L p^N would only meet lambda^{N+1} = 0, so the last segment stops at
         // step N-1 (wavelet has N samples) and its k = N adjoint step skips
         // the gradient term.
         int prev = 0, cur = 1, next = 2;
-        for (int64_t j = k_lo; j <= std::min(k_hi, p.n - 1); ++j) {
+        for (int64_t j = k_lo; !lap_stored && j <= std::min(k_hi, p.n - 1); ++j) {
             T* lap = at<T>(work, p.lap + size_t(j - k_lo) * field);
             FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(j), nullptr, 0, st));
             const int t = prev;
