@@ -239,7 +239,9 @@ int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t 
  *   J = 1/2 sum_{k=1..N} |p^k(plane rec_z) - d_obs^k|^2
  * The gradient is the exact discrete adjoint: dJ/dm = sum_k dt^2 (L p^k)
  * lambda^{k+1}, lambda^k = 2lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1})
- * + residual^k on plane rec_z.  The forward pass keeps the level pair at
+ * + residual^k on plane rec_z, computed in psi = m lambda (psi^k = 2psi^{k+1} -
+ * psi^{k+2} + dt^2 m L psi^{k+1} + m residual^k; dJ/dm = sum_k dt^2 (L p^k)
+ * psi^{k+1} / m).  The forward pass keeps the level pair at
  * every ckpt_every-th step; the backward pass recomputes each segment from
  * it.  With ckpt_every = n_steps (one segment: n_steps fields of workspace,
  * e.g. 134 GB for 256^3 x 2000 FP32) the misfit forward stores every L p^k
@@ -278,9 +280,11 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
                              void* hip_stream);
 /* The single steps: the acoustic step (reference association) plus
  * wavelet[n] at linear cell src_index (< 0: none), writing L p_cur into
- * lap_out (nullable); and the adjoint step, lam_out = 2 lam1 - lam2 +
- * dt^2 L(vel2 lam1) (+ res_plane on plane rec_z when non-null), grad +=
- * dt^2 lap lam1 when lap is non-null, *src_value = lam_out[src_index] when
+ * lap_out (nullable); and the adjoint step in the variable psi = vel2 *
+ * lambda (lambda the adjoint field): psi_out = 2 psi1 - psi2 +
+ * dt^2 vel2 L(psi1) (+ vel2 * res_plane on plane rec_z when non-null), grad +=
+ * dt^2 lap psi1 when lap is non-null (dJ/dvel2 is the sum divided by vel2),
+ * *src_value = psi_out[src_index] / vel2[src_index] (= lambda there) when
  * src_value is non-null. */
 int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const float* vel2, float* p_next,
                              float* lap_out, const stencil_grid* grid, int64_t src_index,
@@ -288,12 +292,12 @@ int stencil_forward_step_f32(const float* p_cur, const float* p_prev, const floa
 int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const double* vel2,
                              double* p_next, double* lap_out, const stencil_grid* grid, int64_t src_index,
                              const double* wavelet, int64_t n, int variant, void* hip_stream);
-int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap,
-                             float* grad, float* lam_out, const float* res_plane, int64_t rec_z,
+int stencil_adjoint_step_f32(const float* psi1, const float* psi2, const float* vel2, const float* lap,
+                             float* grad, float* psi_out, const float* res_plane, int64_t rec_z,
                              const stencil_grid* grid, int64_t src_index, float* src_value, int variant,
                              void* hip_stream);
-int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
-                             double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
+int stencil_adjoint_step_f64(const double* psi1, const double* psi2, const double* vel2, const double* lap,
+                             double* grad, double* psi_out, const double* res_plane, int64_t rec_z,
                              const stencil_grid* grid, int64_t src_index, double* src_value, int variant,
                              void* hip_stream);
 

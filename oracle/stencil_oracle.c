@@ -464,14 +464,16 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
  *   forward  p^{n+1} = 2p^n - p^{n-1} + (dt2*m)*L p^n  (+ wavelet[n] at src)
  *   L p      = (lx + ly) + lz,  la = c0*p*inv + sum_r c_r*(p[+r]+p[-r])*inv
  *   adjoint  lambda^k = 2 lambda^{k+1} - lambda^{k+2} + dt2*L(m lambda^{k+1})
- *                       (+ res^k on plane rec_z)
- *   gradient g += (dt2 * L p^k) * lambda^{k+1}, k = N .. 1
+ *                       (+ res^k on plane rec_z), run in psi = m lambda:
+ *            psi^k = 2 psi^{k+1} - psi^{k+2} + (dt2*m)*L psi^{k+1}
+ *                    (+ m*res^k on plane rec_z)
+ *   gradient gt += (dt2 * L p^k) * psi^{k+1}, k = N .. 1;  g = gt / m
+ *   adjoint field at the source: lambda^k = psi^k / m
  * Pinned by tests/test_oracle_fwi.py: central finite differences of the misfit
  * (FP64), the adjoint dot-product identity, and the forward step against the
  * pinned oracle_wave_step. ---- */
 #define ORACLE_FWI(T, SUF, COEFFS)                                                                       \
-    static void fwi_lap_##SUF(const T* p, const T* m, T* out, int NX, int NY, int NZ, T ix2, T iy2,     \
-                              T iz2) {                                                                  \
+    static void fwi_lap_##SUF(const T* p, T* out, int NX, int NY, int NZ, T ix2, T iy2, T iz2) {       \
         T c[5];                                                                                         \
         COEFFS(c);                                                                                      \
         const ptrdiff_t sy = NX, sz = (ptrdiff_t)NX * NY;                                               \
@@ -479,14 +481,14 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
             for (int iy = R; iy < NY - R; ++iy)                                                         \
                 for (int ix = R; ix < NX - R; ++ix) {                                                   \
                     const ptrdiff_t q = iz * sz + iy * sy + ix;                                         \
-                    const T ctr = m ? m[q] * p[q] : p[q];                                               \
+                    const T ctr = p[q];                                                                 \
                     T lx = c[0] * ctr * ix2, ly = c[0] * ctr * iy2, lz = c[0] * ctr * iz2;              \
                     for (int r = 1; r <= R; ++r) {                                                      \
                         const ptrdiff_t o[3] = {r, r * sy, r * sz};                                     \
                         T a[3], b[3];                                                                   \
                         for (int k = 0; k < 3; ++k) {                                                   \
-                            a[k] = m ? m[q + o[k]] * p[q + o[k]] : p[q + o[k]];                         \
-                            b[k] = m ? m[q - o[k]] * p[q - o[k]] : p[q - o[k]];                         \
+                            a[k] = p[q + o[k]];                                                         \
+                            b[k] = p[q - o[k]];                                                         \
                         }                                                                               \
                         lx += c[r] * (a[0] + b[0]) * ix2;                                               \
                         ly += c[r] * (a[1] + b[1]) * iy2;                                               \
@@ -508,7 +510,7 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
             T* pn = levels + (size_t)(s + 1) * n;                                                       \
             T* lap = laps + (size_t)s * n;                                                              \
             memset(pn, 0, n * sizeof(T));                                                               \
-            fwi_lap_##SUF(p, NULL, lap, NX, NY, NZ, ix2, iy2, iz2);                                     \
+            fwi_lap_##SUF(p, lap, NX, NY, NZ, ix2, iy2, iz2);                                     \
             for (int iz = R; iz < NZ - R; ++iz)                                                         \
                 for (int iy = R; iy < NY - R; ++iy)                                                     \
                     for (int ix = R; ix < NX - R; ++ix) {                                               \
@@ -543,7 +545,7 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
         T* laps = (T*)malloc((size_t)(N + 1) * n * sizeof(T));                                          \
         fwi_levels_##SUF(m, NX, NY, NZ, dx, dy, dz, dt, N, src, wavelet, levels, laps);                 \
         /* L p^N (the forward loop stops at p^N) */                                                     \
-        fwi_lap_##SUF(levels + (size_t)N * n, NULL, laps + (size_t)N * n, NX, NY, NZ, ix2, iy2, iz2);   \
+        fwi_lap_##SUF(levels + (size_t)N * n, laps + (size_t)N * n, NX, NY, NZ, ix2, iy2, iz2);   \
         T* res = (T*)malloc((size_t)N * plane * sizeof(T));                                             \
         double J = 0.0;                                                                                 \
         for (int k = 1; k <= N; ++k)                                                                    \
@@ -558,24 +560,32 @@ void oracle_wave_step16(int fmt, const uint16_t* cur, const int8_t* cur_e, const
         T* lo = (T*)calloc(n, sizeof(T));                                                               \
         T* lq = (T*)calloc(n, sizeof(T));                                                               \
         memset(grad, 0, n * sizeof(T));                                                                 \
+        /* l1, l2, lo: psi^{k+1}, psi^{k+2}, psi^k */                                                   \
         for (int k = N; k >= 1; --k) {                                                                  \
-            fwi_lap_##SUF(l1, m, lq, NX, NY, NZ, ix2, iy2, iz2);                                        \
+            fwi_lap_##SUF(l1, lq, NX, NY, NZ, ix2, iy2, iz2);                                     \
             const T* lp = laps + (size_t)k * n;                                                         \
             for (int iz = R; iz < NZ - R; ++iz)                                                         \
                 for (int iy = R; iy < NY - R; ++iy)                                                     \
                     for (int ix = R; ix < NX - R; ++ix) {                                               \
                         const size_t q = ((size_t)iz * NY + iy) * NX + ix;                              \
-                        T v = (T)2 * l1[q] - l2[q] + dt2 * lq[q];                                       \
-                        if ((size_t)iz == rec_z) v += res[(size_t)(k - 1) * plane + (size_t)iy * nx + ix]; \
+                        T v = (T)2 * l1[q] - l2[q] + dt2 * m[q] * lq[q];                                \
+                        if ((size_t)iz == rec_z)                                                        \
+                            v += m[q] * res[(size_t)(k - 1) * plane + (size_t)iy * nx + ix];            \
                         lo[q] = v;                                                                      \
                         grad[q] += dt2 * lp[q] * l1[q];                                                 \
                     }                                                                                   \
-            if (src_trace) src_trace[k - 1] = lo[src];                                                  \
+            if (src_trace) src_trace[k - 1] = lo[src] / m[src];                                         \
             T* t = l2;                                                                                  \
             l2 = l1;                                                                                    \
             l1 = lo;                                                                                    \
             lo = t;                                                                                     \
         }                                                                                               \
+        for (int iz = R; iz < NZ - R; ++iz)                                                             \
+            for (int iy = R; iy < NY - R; ++iy)                                                         \
+                for (int ix = R; ix < NX - R; ++ix) {                                                   \
+                    const size_t q = ((size_t)iz * NY + iy) * NX + ix;                                  \
+                    grad[q] = grad[q] / m[q];                                                           \
+                }                                                                                       \
         free(levels);                                                                                   \
         free(laps);                                                                                     \
         free(res);                                                                                      \

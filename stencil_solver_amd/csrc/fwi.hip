@@ -9,6 +9,10 @@
 // walls), lambda^{N+1} = lambda^{N+2} = 0:
 //   lambda^k = 2 lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1}) + R^T (R p^k - d^k)
 //   dJ/dm    = sum_k dt^2 (L p^k) lambda^{k+1}
+// is run in the variable psi = m lambda, which turns it into a step of the
+// forward's shape (no m inside the Laplacian, so no m at the halo):
+//   psi^k = 2 psi^{k+1} - psi^{k+2} + dt^2 m L psi^{k+1} + m R^T (R p^k - d^k)
+//   dJ/dm = (sum_k dt^2 (L p^k) psi^{k+1}) / m
 // The forward step here is the reference acoustic step (same association as
 // src/kernels/stencil_naive.cu:75-88) plus the source term, and can also write
 // L p^n for the gradient.  One thread per interior point; neighbour re-reads
@@ -32,17 +36,6 @@ __device__ __forceinline__ T lap_axis(const T* __restrict__ p, size_t c, size_t 
     T lap = Fd<T>::c0 * ctr * inv;
 #pragma unroll
     for (int r = 1; r <= kR; ++r) lap += Fd<T>::c(r) * (p[c + r * s] + p[c - r * s]) * inv;
-    return lap;
-}
-
-// L(m * lam) along one axis, the product formed per neighbour.
-template <class T>
-__device__ __forceinline__ T lap_axis_prod(const T* __restrict__ lam, const T* __restrict__ m, size_t c,
-                                           size_t s, T ctr, T inv) {
-    T lap = Fd<T>::c0 * ctr * inv;
-#pragma unroll
-    for (int r = 1; r <= kR; ++r)
-        lap += Fd<T>::c(r) * (m[c + r * s] * lam[c + r * s] + m[c - r * s] * lam[c - r * s]) * inv;
     return lap;
 }
 
@@ -82,26 +75,36 @@ __global__ __launch_bounds__(kNX* kNY) void fwd_step_kernel(const T* __restrict_
     if (lap_out) lap_out[k.c] = lap;
 }
 
-// lam_out = 2 lam1 - lam2 + dt^2 L(m lam1)  (+ res[y][x] on plane rec_z);
-// grad += dt^2 lap * lam1 (lap = L p^k of the same k); trace[0] = lam_out at src.
+// psi_out = 2 psi1 - psi2 + dt^2 m L psi1  (+ m res[y][x] on plane rec_z);
+// grad += dt^2 lap * psi1 (lap = L p^k of the same k; the sum is divided by m
+// once at the end); trace[0] = psi_out / m at src (the adjoint field there).
 template <class T>
-__global__ __launch_bounds__(kNX* kNY) void adj_step_kernel(const T* __restrict__ lam1, const T* __restrict__ lam2,
+__global__ __launch_bounds__(kNX* kNY) void adj_step_kernel(const T* __restrict__ psi1, const T* __restrict__ psi2,
                                                             const T* __restrict__ m, const T* __restrict__ lap,
-                                                            T* __restrict__ grad, T* __restrict__ lam_out,
+                                                            T* __restrict__ grad, T* __restrict__ psi_out,
                                                             const T* __restrict__ res, int rec_z, int nx, int ny,
                                                             Scalars<T> s, long long src, T* __restrict__ trace) {
     Cell k;
     if (!interior_cell(nx, ny, k)) return;
-    const T l1 = lam1[k.c];
-    const T qc = m[k.c] * l1;
-    const T lx = lap_axis_prod(lam1, m, k.c, 1, qc, s.inv_dx2);
-    const T ly = lap_axis_prod(lam1, m, k.c, k.sy, qc, s.inv_dy2);
-    const T lz = lap_axis_prod(lam1, m, k.c, k.sz, qc, s.inv_dz2);
-    T v = T(2) * l1 - lam2[k.c] + s.dt2 * (lx + ly + lz);
-    if (res && k.iz == rec_z) v += res[static_cast<size_t>(k.iy) * k.sy + k.ix];
-    lam_out[k.c] = v;
-    if (lap) grad[k.c] += s.dt2 * lap[k.c] * l1;
-    if (trace && static_cast<long long>(k.c) == src) trace[0] = v;
+    const T pc = psi1[k.c];
+    const T lx = lap_axis(psi1, k.c, 1, pc, s.inv_dx2);
+    const T ly = lap_axis(psi1, k.c, k.sy, pc, s.inv_dy2);
+    const T lz = lap_axis(psi1, k.c, k.sz, pc, s.inv_dz2);
+    const T lp = lx + ly + lz;
+    const T mc = m[k.c];
+    T v = T(2) * pc - psi2[k.c] + s.dt2 * mc * lp;
+    if (res && k.iz == rec_z) v += mc * res[static_cast<size_t>(k.iy) * k.sy + k.ix];
+    psi_out[k.c] = v;
+    if (lap) grad[k.c] += s.dt2 * lap[k.c] * pc;
+    if (trace && static_cast<long long>(k.c) == src) trace[0] = v / mc;
+}
+
+// grad = acc / m on interior cells (the halo keeps its zeros).
+template <class T>
+__global__ __launch_bounds__(kNX* kNY) void divide_kernel(T* __restrict__ g, const T* __restrict__ m, int nx, int ny) {
+    Cell k;
+    if (!interior_cell(nx, ny, k)) return;
+    g[k.c] = g[k.c] / m[k.c];
 }
 
 inline dim3 grid_of(const Dims& d) {
@@ -165,11 +168,19 @@ hipError_t fwd_step_t(const T* p, const T* pp, const T* m, T* pn, T* lap_out, co
 }
 
 template <class T>
-hipError_t adj_step_t(const T* lam1, const T* lam2, const T* m, const T* lap, T* grad, T* lam_out, const T* res,
+hipError_t adj_step_t(const T* psi1, const T* psi2, const T* m, const T* lap, T* grad, T* psi_out, const T* res,
                       int rec_z, const Dims& d, const Scalars<T>& s, long long src, T* trace, hipStream_t st) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
-    hipLaunchKernelGGL(adj_step_kernel<T>, grid_of(d), dim3(kNX, kNY, 1), 0, st, lam1, lam2, m, lap, grad, lam_out,
+    hipLaunchKernelGGL(adj_step_kernel<T>, grid_of(d), dim3(kNX, kNY, 1), 0, st, psi1, psi2, m, lap, grad, psi_out,
                        res, rec_z, static_cast<int>(d.nx), static_cast<int>(d.ny), s, src, trace);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t divide_t(T* g, const T* m, const Dims& d, hipStream_t st) {
+    if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
+    hipLaunchKernelGGL(divide_kernel<T>, grid_of(d), dim3(kNX, kNY, 1), 0, st, g, m, static_cast<int>(d.nx),
+                       static_cast<int>(d.ny));
     return hipGetLastError();
 }
 
@@ -200,16 +211,18 @@ hipError_t fwi_fwd_step_f64(const double* p, const double* pp, const double* m, 
                             hipStream_t st) {
     return fwd_step_t(p, pp, m, pn, lap_out, d, s, src, wavelet, n, st);
 }
-hipError_t fwi_adj_step_f32(const float* lam1, const float* lam2, const float* m, const float* lap, float* grad,
-                            float* lam_out, const float* res, int rec_z, const Dims& d, const Scalars<float>& s,
+hipError_t fwi_adj_step_f32(const float* psi1, const float* psi2, const float* m, const float* lap, float* grad,
+                            float* psi_out, const float* res, int rec_z, const Dims& d, const Scalars<float>& s,
                             long long src, float* trace, hipStream_t st) {
-    return adj_step_t(lam1, lam2, m, lap, grad, lam_out, res, rec_z, d, s, src, trace, st);
+    return adj_step_t(psi1, psi2, m, lap, grad, psi_out, res, rec_z, d, s, src, trace, st);
 }
-hipError_t fwi_adj_step_f64(const double* lam1, const double* lam2, const double* m, const double* lap,
-                            double* grad, double* lam_out, const double* res, int rec_z, const Dims& d,
+hipError_t fwi_adj_step_f64(const double* psi1, const double* psi2, const double* m, const double* lap,
+                            double* grad, double* psi_out, const double* res, int rec_z, const Dims& d,
                             const Scalars<double>& s, long long src, double* trace, hipStream_t st) {
-    return adj_step_t(lam1, lam2, m, lap, grad, lam_out, res, rec_z, d, s, src, trace, st);
+    return adj_step_t(psi1, psi2, m, lap, grad, psi_out, res, rec_z, d, s, src, trace, st);
 }
+hipError_t fwi_divide_f32(float* g, const float* m, const Dims& d, hipStream_t st) { return divide_t(g, m, d, st); }
+hipError_t fwi_divide_f64(double* g, const double* m, const Dims& d, hipStream_t st) { return divide_t(g, m, d, st); }
 hipError_t fwi_residual_f32(const float* syn, const float* obs, float* res, size_t n, double* partial, double* out,
                             hipStream_t st) {
     return residual_t(syn, obs, res, n, partial, out, st);

@@ -8,9 +8,10 @@
 // R p^k - d^k in place and J = 1/2 |residual|^2 (FP64 reduction).
 // Backward pass, segments last to first: the forward steps of segment s are
 // re-run from its checkpoint, writing L p^k for k = sC+1 .. min((s+1)C, N-1);
-// then the adjoint steps k = min((s+1)C, N) .. sC+1 consume them:
-//   lambda^k = 2 lambda^{k+1} - lambda^{k+2} + dt^2 L(m lambda^{k+1}) + res^k
-//   grad    += dt^2 (L p^k) lambda^{k+1}
+// then the adjoint steps k = min((s+1)C, N) .. sC+1 consume them, in the
+// variable psi = m lambda:
+//   psi^k = 2 psi^{k+1} - psi^{k+2} + dt^2 m L psi^{k+1} + m res^k
+//   grad += dt^2 (L p^k) psi^{k+1},   and at the end grad /= m.
 // Device memory: all of it is the caller's workspace (stencil_fwi_workspace_bytes):
 // 3 wavefield levels, 3 adjoint levels, 2 * ceil(N/C) checkpoint fields, C
 // Laplacian fields, N receiver planes and the reduction scratch.
@@ -38,6 +39,8 @@ hipError_t fwi_residual_f32(const float*, const float*, float*, size_t, double*,
 hipError_t fwi_residual_f64(const double*, const double*, double*, size_t, double*, double*, hipStream_t);
 hipError_t fill_f32(float*, size_t, float, hipStream_t);
 hipError_t fill_f64(double*, size_t, double, hipStream_t);
+hipError_t fwi_divide_f32(float*, const float*, const Dims&, hipStream_t);
+hipError_t fwi_divide_f64(double*, const double*, const Dims&, hipStream_t);
 
 }  // namespace stencil::hipk
 
@@ -159,6 +162,9 @@ struct Ops<float> {
         return fwi_residual_f32(a, b, r, n, part, out, st);
     }
     static hipError_t fill(float* p, size_t n, hipStream_t st) { return fill_f32(p, n, 0.f, st); }
+    static hipError_t divide(float* g, const float* m, const Dims& d, hipStream_t st) {
+        return fwi_divide_f32(g, m, d, st);
+    }
 };
 template <>
 struct Ops<double> {
@@ -180,6 +186,9 @@ struct Ops<double> {
         return fwi_residual_f64(a, b, r, n, part, out, st);
     }
     static hipError_t fill(double* p, size_t n, hipStream_t st) { return fill_f64(p, n, 0.0, st); }
+    static hipError_t divide(double* g, const double* m, const Dims& d, hipStream_t st) {
+        return fwi_divide_f64(g, m, d, st);
+    }
 };
 
 #define FWI_TRY(x)                                      \
@@ -261,7 +270,7 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     const T* res = at<T>(work, p.traces);
     FWI_TRY(Ops<T>::fill(grad, p.cells, st));
     for (auto* l : lam) FWI_TRY(Ops<T>::fill(l, p.cells, st));
-    int l1 = 0, l2 = 1, lo = 2;  // lambda^{k+1}, lambda^{k+2}, lambda^k
+    int l1 = 0, l2 = 1, lo = 2;  // psi^{k+1}, psi^{k+2}, psi^k (psi = m lambda)
     const bool lap_stored = p.C == p.n;  // the misfit forward wrote every L p^k
     for (int64_t sg = p.S - 1; sg >= 0; --sg) {
         const int64_t k_lo = sg * p.C + 1;
@@ -295,6 +304,8 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
             lo = t;
         }
     }
+    // grad held sum_k dt^2 (L p^k) psi^{k+1}; lambda = psi / m
+    FWI_TRY(Ops<T>::divide(grad, m, p.d, st));
     return STENCIL_OK;
 }
 
@@ -390,14 +401,14 @@ int stencil_forward_step_f64(const double* p_cur, const double* p_prev, const do
                              int64_t n, int variant, void* stream) {
     return fwd_step_api(p_cur, p_prev, vel2, p_next, lap_out, grid, src_index, wavelet, n, variant, stream);
 }
-int stencil_adjoint_step_f32(const float* lam1, const float* lam2, const float* vel2, const float* lap, float* grad,
-                             float* lam_out, const float* res_plane, int64_t rec_z, const stencil_grid* grid,
+int stencil_adjoint_step_f32(const float* psi1, const float* psi2, const float* vel2, const float* lap, float* grad,
+                             float* psi_out, const float* res_plane, int64_t rec_z, const stencil_grid* grid,
                              int64_t src_index, float* src_value, int variant, void* stream) {
-    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
+    return adj_step_api(psi1, psi2, vel2, lap, grad, psi_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
 }
-int stencil_adjoint_step_f64(const double* lam1, const double* lam2, const double* vel2, const double* lap,
-                             double* grad, double* lam_out, const double* res_plane, int64_t rec_z,
+int stencil_adjoint_step_f64(const double* psi1, const double* psi2, const double* vel2, const double* lap,
+                             double* grad, double* psi_out, const double* res_plane, int64_t rec_z,
                              const stencil_grid* grid, int64_t src_index, double* src_value, int variant, void* stream) {
-    return adj_step_api(lam1, lam2, vel2, lap, grad, lam_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
+    return adj_step_api(psi1, psi2, vel2, lap, grad, psi_out, res_plane, rec_z, grid, src_index, src_value, variant, stream);
 }
 }  // extern "C"

@@ -42,9 +42,9 @@ using dev::Fd;
 using dev::kR;
 
 // kWaveFwi: the exact wave step plus a point source and an optional L p
-// output (FWI forward / recompute); kAdjoint: the adjoint step
-// lambda_out = 2 lambda1 - lambda2 + dt^2 L(m lambda1) with the residual on
-// the receiver plane and the gradient update (FWI backward).
+// output (FWI forward / recompute); kAdjoint: the adjoint step in psi = m lambda,
+// psi_out = 2 psi1 - psi2 + dt^2 m L psi1 with m times the residual on the
+// receiver plane and the gradient update (FWI backward).
 enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3, kWaveFwi = 4, kAdjoint = 5 };
 
 struct ZqArgs {
@@ -67,11 +67,12 @@ struct ZqArgs {
     // FWI modes (FP32).  kWaveFwi: cur = p^n, prev = p^{n-1}, vel2 = m,
     // next = p^{n+1}, lap_out = L p^n (nullable), src_val = &wavelet[n]
     // (nullable) added at cell (src_z, src_y, src_x), trace (nullable) = the
-    // nx*ny receiver plane that receives p^{n+1} on plane rec_z.  kAdjoint: cur =
-    // lambda^{k+1}, aux = m, prev = lambda^{k+2}, vel2 = L p^k (nullable: no
-    // gradient term), grad updated in place, next = lambda^k, res = residual
-    // plane (nullable) added on plane rec_z, trace (nullable) = lambda^k at
-    // the source cell.
+    // nx*ny receiver plane that receives p^{n+1} on plane rec_z.  kAdjoint, in
+    // psi = m lambda: cur = psi^{k+1}, prev = psi^{k+2}, vel2 = m, aux = L p^k
+    // (nullable: no gradient term), grad += dt^2 (L p^k) psi^{k+1} in place,
+    // next = psi^k = 2 psi^{k+1} - psi^{k+2} + dt^2 m L psi^{k+1}, plus m times
+    // the residual plane res (nullable) on plane rec_z, trace (nullable) =
+    // psi^k / m at the source cell.
     const void* aux;
     void* lap_out;
     void* grad;
@@ -199,7 +200,7 @@ constexpr int min_waves(int txv, int ty, int fmt, int mode) {
     const bool shuffle_x = txv == 64 && fmt == dev::kF32;  // x neighbours by lane shifts: no column items
     const int halo_per = (2 * dev::kR * txv + (shuffle_x ? 0 : 2 * ty) + threads - 1) / threads;
     if (threads == 768 || halo_per >= 3 || (halo_per == 2 && (fmt == dev::kBfp16 || threads == 256))) return 3;
-    if (mode == kAdjoint && threads != 1024) return 3;  // m and lambda staged, three more streams
+    if (mode == kAdjoint && threads != 1024) return 3;  // two more streams (L p, gradient)
     if (fmt == dev::kF32 && prefetch_depth(txv, ty, mode) > 2 && halo_per >= 2) return 3;
     return 4;
 }
@@ -360,8 +361,8 @@ void zq_kernel(ZqArgs a) {
     // holds plane zz (x and y neighbours), zq[k] is plane zz - 4 + k.
     // dt2v multiplies vv in the fused forms (dt^2, or dt^2 * 2^k when vv holds
     // the raw mantissas of a block-FP16 vel2 block).
-    // lc: lambda^{k+1} at the point (kAdjoint; the staged values are m lambda);
-    // lap4 receives L p (kWaveFwi).
+    // lc: L p^k at the point (kAdjoint's gradient term); lap4 receives L p
+    // (kWaveFwi).
     // The x neighbour float4s of this lane's centre (plane in LDS buffer b).
     // Called by every lane, outside any divergent branch: the lane shifts
     // read the adjacent lanes' registers, which must be active.
@@ -407,9 +408,7 @@ void zq_kernel(ZqArgs a) {
                 }
                 if constexpr (kHeat) {
                     res = p + a.coef * (lx + ly + lz);
-                } else if constexpr (kAdj) {
-                    res = 2.f * comp(lc, i) - comp(pv, i) + a.s.dt2 * (lx + ly + lz);
-                } else {
+                } else {  // wave, FWI forward, adjoint in psi (vv = m)
                     res = 2.f * p - comp(pv, i) + a.s.dt2 * comp(vv, i) * (lx + ly + lz);
                     if constexpr (kFwd) dev::set_comp(lap4, i, lx + ly + lz);
                 }
@@ -456,17 +455,19 @@ void zq_kernel(ZqArgs a) {
                         store4<FMT, NT, true>(prs(a.lap_out, zz, plane_bytes), prs(a.lap_out, zz, 0),
                                               own_b, 0, lap4, 1.f, own_out, false);
                 } else {
-                    if (a.res && zz == a.rec_z && own_out) {
+                    if (a.res && zz == a.rec_z && own_out) {  // + m * residual
                         const float4 r = *reinterpret_cast<const float4*>(a.res + own_off);
-                        out = make_float4(out.x + r.x, out.y + r.y, out.z + r.z, out.w + r.w);
+                        out = make_float4(out.x + vv.x * r.x, out.y + vv.y * r.y, out.z + vv.z * r.z,
+                                          out.w + vv.w * r.w);
                     }
-                    if (a.vel2) {  // gradient: grad += dt^2 * (L p^k) * lambda^{k+1}
-                        const float4 g = make_float4(gr.x + a.s.dt2 * vv.x * lc.x, gr.y + a.s.dt2 * vv.y * lc.y,
-                                                     gr.z + a.s.dt2 * vv.z * lc.z, gr.w + a.s.dt2 * vv.w * lc.w);
+                    if (a.aux) {  // gradient: grad += dt^2 * (L p^k) * psi^{k+1}
+                        const float4 c = zq[kR];
+                        const float4 g = make_float4(gr.x + a.s.dt2 * lc.x * c.x, gr.y + a.s.dt2 * lc.y * c.y,
+                                                     gr.z + a.s.dt2 * lc.z * c.z, gr.w + a.s.dt2 * lc.w * c.w);
                         store4<FMT, NT, true>(prs(a.grad, zz, plane_bytes), prs(a.grad, zz, 0),
                                               own_b, 0, g, 1.f, own_out, false);
                     }
-                    if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x);
+                    if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x) / comp(vv, a.src_x - x);
                 }
             }
             store4<FMT, NT, kExact>(prs(next_b, zz, plane_bytes),
@@ -477,23 +478,10 @@ void zq_kernel(ZqArgs a) {
 
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
     const char* const aux_b = static_cast<const char*>(a.aux);
-    // kAdjoint stages m * lambda (the product the adjoint Laplacian acts on).
-    auto staged = [&](int z, unsigned boff) -> float4 {
-        if constexpr (FMT == dev::kF32) {
-            const float4 l = raw_at(cur_b, z, boff);
-            if constexpr (kAdj) {
-                const float4 m = raw_at(aux_b, z, boff);
-                return make_float4(m.x * l.x, m.y * l.y, m.z * l.z, m.w * l.w);
-            }
-            return l;
-        } else {
-            return make_float4(0.f, 0.f, 0.f, 0.f);  // 16-bit storage decodes in its own loop
-        }
-    };
     float4 q[9];
     if constexpr (FMT == dev::kF32) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = staged(z_begin - kR + k, own_ld);
+        for (int k = 0; k < 8; ++k) q[k] = raw_at(cur_b, z_begin - kR + k, own_ld);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -503,107 +491,73 @@ void zq_kernel(ZqArgs a) {
 
     if constexpr (FMT == dev::kF32) {
         // FP32: every load is issued kPD steps ahead: the queue refill for
-        // plane zz+4+kPD (kAdjoint: lambda and m, multiplied when they enter
-        // the queue), halo and streams for plane zz+kPD; stage d of each FIFO
-        // holds plane zz+d (queue refills: zz+4+d).  (kAdjoint also prefetches
-        // m for the halo, lambda1 at the point, and the gradient; its vel2
-        // stream is L p^k, absent when vel2 is null.)
+        // plane zz+4+kPD, halo and streams for plane zz+kPD; stage d of each
+        // FIFO holds plane zz+d (queue refills: zz+4+d).  (kAdjoint streams
+        // vel2 = m, and L p^k and the gradient when aux is non-null.)
         constexpr int kPD = prefetch_depth(TXV, TY, MODE);
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool lap_in = kAdj && a.vel2 != nullptr;
-        float4 rq[kPD], rqm[kPD], xeq[kPD], xemq[kPD];
-        float4 hq[kPD][kHaloPer], hmq[kPD][kHaloPer];
+        const bool lap_in = kAdj && a.aux != nullptr;
+        float4 rq[kPD], xeq[kPD];
+        float4 hq[kPD][kHaloPer];
         float4 pvq[kPD], vvq[kPD], lcq[kPD], grq[kPD];
         // Loads of plane z into FIFO stage d (queue refill: plane z + kR).
-        auto fetch = [&](int z, float4& r, float4& rm, float4& xe, float4& xem, float4 (&h)[kHaloPer],
-                         float4 (&hm)[kHaloPer], float4& pv, float4& vv, float4& lc, float4& gr) {
-            const int zr = min(z + kR, z_last);
-            r = raw_at(cur_b, zr, own_ld);
-            rm = xe = xem = zero4;
-            if constexpr (kAdj) rm = raw_at(aux_b, zr, own_ld);
-            if constexpr (kShuffleX) {
-                xe = raw_at(cur_b, z, xe_ld);
-                if constexpr (kAdj) xem = raw_at(aux_b, z, xe_ld);
-            }
+        auto fetch = [&](int z, float4& r, float4& xe, float4 (&h)[kHaloPer], float4& pv, float4& vv, float4& lc,
+                         float4& gr) {
+            r = raw_at(cur_b, min(z + kR, z_last), own_ld);
+            xe = zero4;
+            if constexpr (kShuffleX) xe = raw_at(cur_b, z, xe_ld);
 #pragma unroll
-            for (int k = 0; k < kHaloPer; ++k) {
-                h[k] = raw_at(cur_b, z, h_ld[k]);
-                hm[k] = zero4;
-                if constexpr (kAdj) hm[k] = raw_at(aux_b, z, h_ld[k]);
-            }
+            for (int k = 0; k < kHaloPer; ++k) h[k] = raw_at(cur_b, z, h_ld[k]);
             pv = vv = lc = gr = zero4;
             if constexpr (!kHeat) {
                 pv = raw_stream(prev_b, z);
-                if (!kAdj || lap_in) vv = raw_stream(vel_b, z);
+                vv = raw_stream(vel_b, z);
             }
             if constexpr (kAdj) {
-                lc = raw_at(cur_b, z, own_ld);
-                if (lap_in) gr = raw_stream(static_cast<const char*>(a.grad), z);
+                if (lap_in) {
+                    lc = raw_stream(aux_b, z);
+                    gr = raw_stream(static_cast<const char*>(a.grad), z);
+                }
             }
         };
 #pragma unroll
         for (int d = 0; d < kPD; ++d)
-            fetch(min(z_begin + d, z_last), rq[d], rqm[d], xeq[d], xemq[d], hq[d], hmq[d], pvq[d], vvq[d], lcq[d],
-                  grq[d]);
+            fetch(min(z_begin + d, z_last), rq[d], xeq[d], hq[d], pvq[d], vvq[d], lcq[d], grq[d]);
         for (int zb = z_begin; zb < z_end; zb += 9) {
             // The 9 steps are expanded at compile time (j is a constant in
             // every queue index), so the queue arrays stay in registers.
             dev::static_for<9>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 const int zz = zb + j;
-                if constexpr (kAdj)
-                    q[(j + 8) % 9] = make_float4(rqm[0].x * rq[0].x, rqm[0].y * rq[0].y, rqm[0].z * rq[0].z,
-                                                 rqm[0].w * rq[0].w);
-                else
-                    q[(j + 8) % 9] = rq[0];
-                float4 r_n, rm_n, xe_n, xem_n, h_n[kHaloPer], hm_n[kHaloPer], pv_n, vv_n, lc_n, gr_n;
-                fetch(min(zz + kPD, z_last), r_n, rm_n, xe_n, xem_n, h_n, hm_n, pv_n, vv_n, lc_n, gr_n);
+                q[(j + 8) % 9] = rq[0];
+                float4 r_n, xe_n, h_n[kHaloPer], pv_n, vv_n, lc_n, gr_n;
+                fetch(min(zz + kPD, z_last), r_n, xe_n, h_n, pv_n, vv_n, lc_n, gr_n);
                 const int b = (zz - z_begin) & 1;
                 lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
-                    if (tid + k * kThreads < kHalo) {
-                        float4 hv = hq[0][k];
-                        if constexpr (kAdj)
-                            hv = make_float4(hmq[0][k].x * hv.x, hmq[0][k].y * hv.y, hmq[0][k].z * hv.z,
-                                             hmq[0][k].w * hv.w);
-                        lds[b][h_row[k]][h_col[k]] = hv;
-                    }
+                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = hq[0][k];
                 __syncthreads();
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                float4 edge = xeq[0];
-                if constexpr (kAdj)
-                    edge = make_float4(xemq[0].x * edge.x, xemq[0].y * edge.y, xemq[0].z * edge.z,
-                                       xemq[0].w * edge.w);
-                finish(zz, b, zq, edge, pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0]);
+                finish(zz, b, zq, xeq[0], pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0]);
                 // FIFO shift (register renaming under the unroll)
 #pragma unroll
                 for (int d = 0; d + 1 < kPD; ++d) {
                     rq[d] = rq[d + 1];
-                    rqm[d] = rqm[d + 1];
                     xeq[d] = xeq[d + 1];
-                    xemq[d] = xemq[d + 1];
 #pragma unroll
-                    for (int k = 0; k < kHaloPer; ++k) {
-                        hq[d][k] = hq[d + 1][k];
-                        hmq[d][k] = hmq[d + 1][k];
-                    }
+                    for (int k = 0; k < kHaloPer; ++k) hq[d][k] = hq[d + 1][k];
                     pvq[d] = pvq[d + 1];
                     vvq[d] = vvq[d + 1];
                     lcq[d] = lcq[d + 1];
                     grq[d] = grq[d + 1];
                 }
                 rq[kPD - 1] = r_n;
-                rqm[kPD - 1] = rm_n;
                 xeq[kPD - 1] = xe_n;
-                xemq[kPD - 1] = xem_n;
 #pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) {
-                    hq[kPD - 1][k] = h_n[k];
-                    hmq[kPD - 1][k] = hm_n[k];
-                }
+                for (int k = 0; k < kHaloPer; ++k) hq[kPD - 1][k] = h_n[k];
                 pvq[kPD - 1] = pv_n;
                 vvq[kPD - 1] = vv_n;
                 lcq[kPD - 1] = lc_n;
@@ -949,16 +903,16 @@ hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int st
     return launch_zq(a, d, shape, kWaveFwi, st, dev::kF32, n_shots);
 }
 
-hipError_t adjoint_zq_f32(const float* lam1, const float* lam2, const float* m, const float* lap, float* grad,
-                          float* lam_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, float* grad,
+                          float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
                           float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st) {
     ZqArgs a{};
-    a.cur = lam1;
-    a.aux = m;
-    a.prev = lam2;
-    a.vel2 = lap;
+    a.cur = psi1;
+    a.prev = psi2;
+    a.vel2 = m;
+    a.aux = lap;
     a.grad = grad;
-    a.next = lam_out;
+    a.next = psi_out;
     a.res = res;
     a.rec_z = rec_z;
     a.trace = trace;
