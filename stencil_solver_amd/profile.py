@@ -96,6 +96,9 @@ def main(argv=None):
     ap.add_argument("--algorithmic-bytes", type=float, default=None,
                     help="algorithmic bytes per launch of that kernel (for the traffic ratio)")
     ap.add_argument("--no-pmc", action="store_true", help="trace pass only")
+    ap.add_argument("--rate", action="append", default=[], metavar="REGEX=BYTES",
+                    help="algorithmic bytes per launch of the kernels matching REGEX: adds their "
+                         "GB/s and fraction of the 8 TB/s peak to the kernel table (repeatable)")
     ap.add_argument("--copy-n", type=int, default=None, help="cells per axis of the calibration copy (--n of bench)")
     ap.add_argument("--timeout", type=int, default=600)
     ap.add_argument("--keep", action="store_true", help="keep raw per-dispatch files")
@@ -125,16 +128,38 @@ def main(argv=None):
 
     kname, _ = _pick({r["name"]: r["avg_ns"] * r["calls"] for r in stats}, a.kernel)
     krow = next((r for r in stats if r["name"] == kname), None)
+    import shlex
+
+    wrapper = "python -m stencil_solver_amd.profile " + " ".join(shlex.quote(x) for x in argv)
     lines = [f"# rocprofv3 summary: {a.name}", "",
-             "Made by `python -m stencil_solver_amd.profile` from the repository root on one MI355X; the passes it ran:",
-             "", "```"]
+             "Made on one MI355X from the repository root by", "", "```", wrapper, "```", "",
+             "which ran these passes:", "", "```"]
     for k, c in cmds.items():
         lines.append(" ".join(c).replace(str(ROOT) + "/", ""))
-    lines += ["```", "", "## Kernels (trace pass)", "", "| Kernel | Calls | Average us | Min us | Max us |",
-              "|---|---|---|---|---|"]
-    for r in stats[:12]:
-        lines.append(f"| `{r['name'][:110]}` | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | "
-                     f"{r['max_ns'] / 1e3:.1f} |")
+    rates = []
+    for spec in a.rate:
+        pat, _, nb = spec.rpartition("=")
+        rates.append((re.compile(pat), float(nb)))
+
+    def rate_cells(name, avg_ns):
+        for pat, nb in rates:
+            if pat.search(name):
+                gbs = nb / avg_ns
+                return f" {nb / 1e9:.4f} | {gbs:.0f} | {gbs / 8000.0:.3f} |"
+        return " | | |"
+
+    lines += ["```", "", "## Kernels (trace pass)", ""]
+    if rates:
+        lines += ["Algorithmic bytes per launch as given by `--rate`; GB/s = those bytes / average duration; "
+                  "fraction of the 8 TB/s HBM peak.", "",
+                  "| Kernel | Calls | Average us | Min us | Max us | Algorithmic GB | GB/s | of 8 TB/s |",
+                  "|---|---|---|---|---|---|---|---|"]
+    else:
+        lines += ["| Kernel | Calls | Average us | Min us | Max us |", "|---|---|---|---|---|"]
+    for r in stats[:24 if rates else 12]:
+        row = (f"| `{r['name'][:110]}` | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | "
+               f"{r['max_ns'] / 1e3:.1f} |")
+        lines.append(row + (rate_cells(r["name"], r["avg_ns"]) if rates else ""))
     traffic = {"kernel": kname, "avg_kernel_ns": krow["avg_ns"] if krow else None}
     if counters:
         n = a.copy_n
