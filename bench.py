@@ -265,6 +265,30 @@ def main():
         tms = e0.elapsed_time(e1) / 20
         extras["stream_3r1w_gbs"] = round(16.0 * dst.numel() / (tms * 1e-3) / 1e9, 1)
         del src3, dst
+        # One rank's share of an 8-way Z-slab split of this grid (the design
+        # for more GPUs): 64 owned planes plus the 4 exchanged planes on each
+        # face, the same tiled kernel at that shape (no exchange here).
+        z0 = n // 2 - 36
+        sgrid = ss.GridDims(n, n, 72, h, h, h, dt)
+        sp = p0[z0:z0 + 72].contiguous()
+        sv = vel2[z0:z0 + 72].contiguous()
+        sb = [sp.clone(), sp.clone(), torch.zeros_like(sp)]
+        if not args.no_tune:
+            extras["slab_shape"] = ss.tune_tiled(sb[1], sb[0], sv, sb[2], sgrid, reps=5)
+        i = 0
+        for it in range(520):
+            if it == 20:
+                torch.cuda.synchronize()
+                e0.record()
+            ss.launch_tiled(sb[(i + 1) % 3], sb[i], sv, sb[(i + 2) % 3], sgrid)
+            i = (i + 1) % 3
+        e1.record()
+        torch.cuda.synchronize()
+        sms = e0.elapsed_time(e1) / 500
+        spts = (n - 8) * (n - 8) * 64
+        extras["slab_8way_ms"] = round(sms, 5)
+        extras["slab_8way_gpts"] = round(spts / (sms * 1e-3) / 1e9, 1)
+        del sp, sv, sb
         result["extras"] = extras
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
