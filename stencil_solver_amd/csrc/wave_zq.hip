@@ -9,8 +9,8 @@
 // on x = 0 (not on the first interior column), so a tile row is whole 128-byte
 // lines and no line is written by two workgroups.
 //   * z neighbours: a 9-deep register queue of float4 per thread (planes
-//     z-4..z+4), refilled with one load per plane; the z loop is unrolled by 9
-//     so queue slots are renamed, never moved.
+//     z-4..z+4), refilled with one load per plane; the 9 steps of a group are
+//     expanded at compile time, so queue slots are renamed, never moved.
 //   * x / y neighbours: plane z is staged in LDS as (TY+8) rows x (TXV+2)
 //     float4, written from the queue centre plus the halo float4 (top/bottom 4
 //     rows, left/right 4 columns; corners are never read by the star stencil).
@@ -20,14 +20,15 @@
 //     row's end lanes loading the float4 past the tile edge themselves; LDS
 //     then holds rows only.
 //   * p_prev, vel2 and p_next are touched once per step.  Every load is issued
-//     one step ahead: the queue refill for plane z+5 and the stream and halo
-//     loads for plane z+1 go out before plane z is computed (gfx950 barriers
-//     do not drain outstanding global loads).  With NT they use
-//     non-temporal accesses so they do not displace p_cur lines in L2.
+//     ahead (one plane; three for the heat step): the queue refill for plane
+//     z+5 and the stream and halo loads for plane z+1 go out before plane z is
+//     computed (gfx950 barriers do not drain outstanding global loads).  With
+//     NT they use non-temporal accesses so they do not displace p_cur lines
+//     in L2.
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
-// write p_next once = 16 B (halo re-reads are L2 hits; the tile order keeps
-// neighbouring tiles on one XCD).  Chained steps alternate the z direction
-// (ZqArgs::mirror).
+// write p_next once = 16 B (measured 1.05x: halo re-reads are L2 hits, the
+// tile order keeping neighbouring tiles on one XCD).  Chained steps alternate
+// the z direction (ZqArgs::mirror).
 #include <type_traits>
 #include <cstdlib>
 
