@@ -11,6 +11,7 @@
 #include <stdexcept>
 
 #include "stencil/types.hpp"
+#include "launchers.hpp"
 #include "stencil_capi.h"
 
 namespace stencil {
@@ -62,7 +63,7 @@ void write_results_json(const std::string& path, const std::vector<PerfResult>& 
         if (stencil_tuned_shape(&g, 0, &shape) >= 0) {
             tile_x = 4 * shape.tile_x4;
             tile_y = shape.tile_y;
-            pencil_z = shape.n_chunks > 0 ? int((n - 8 + shape.n_chunks - 1) / shape.n_chunks) : int(n - 8);
+            pencil_z = std::min(int(n - 8), hipk::zq_chunk_planes(int64_t(n - 8), shape.n_chunks));
         }
     }
     std::ostringstream js;

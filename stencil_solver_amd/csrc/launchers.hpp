@@ -47,6 +47,8 @@ bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
 bool zq_shape_valid(int txv, int ty);
 // Fills unset fields of a shape.
 ZqShape zq_resolve(const Dims& d, ZqShape s, int kind);
+// z-planes per chunk of the tiled kernel for n_chunks chunks of a column.
+int zq_chunk_planes(int64_t interior_planes, int n_chunks);
 
 // Autotuner cache (tune.cpp): the fastest shape measured for a grid size;
 // kind 0 = wave step, 1 = heat step.

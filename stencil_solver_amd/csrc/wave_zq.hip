@@ -758,9 +758,7 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     a.nz = static_cast<int>(d.nz);
     a.tiles_x = static_cast<int>((d.nx + 4 * shape.txv - 1) / (4 * shape.txv));
     a.tiles_y = static_cast<int>((d.ny - 2 * kR + shape.ty - 1) / shape.ty);
-    // Chunks are whole groups of 9 planes (the kernel's unroll), so only the
-    // last chunk of a column runs idle steps.
-    a.chunk = static_cast<int>(((inz + shape.n_chunks - 1) / shape.n_chunks + 8) / 9 * 9);
+    a.chunk = zq_chunk_planes(inz, shape.n_chunks);
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = shape.xcd_remap != 0 ? 1 : 0;
@@ -779,6 +777,13 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
 }
 
 }  // namespace
+
+// Chunks are whole groups of 9 planes (the kernel's step group), so only the
+// last chunk of a column runs idle steps.
+int zq_chunk_planes(int64_t interior_planes, int n_chunks) {
+    if (n_chunks < 1) n_chunks = 1;
+    return static_cast<int>(((interior_planes + n_chunks - 1) / n_chunks + 8) / 9 * 9);
+}
 
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs) {
     if (d.nx % 4 != 0) return false;

@@ -1,14 +1,23 @@
 // Host-side unit tests re-hosting the reference's CPU-labelled suites on this
 // library's C++ API (no device needed):
 //   tests/unit/test_fd_coeffs.cpp, test_grid.cpp, test_solver_cpu.cpp,
-//   test_timer.cpp  (the gtest assertions, same names and tolerances).
+//   test_timer.cpp, test_io.cpp  (the gtest assertions, same names and
+//   tolerances).
 #include <chrono>
 #include <cmath>
 #include <thread>
 #include <vector>
 
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <unistd.h>
+
 #include "mini_test.hpp"
 #include "stencil/config.hpp"
+#include "stencil/io.hpp"
+#include "stencil_capi.h"
 #include "stencil/grid.hpp"
 #include "stencil/timer.hpp"
 #include "stencil/types.hpp"
@@ -395,3 +404,109 @@ TEST(BandwidthFlops, PositiveValues) {
 }
 
 MT_MAIN()
+
+// ── test_io.cpp:45-132 ───────────────────────────────────────────────────────
+// The reference checks tile_x / tile_y / pencil_z against its compile-time
+// macros; here they are the launch shape the tiled kernel uses on the largest
+// grid of the run (cells, rows, z-chunk planes), read back through the C-ABI.
+namespace {
+
+std::string io_path() { return "/tmp/stencil_test_io_" + std::to_string(getpid()) + ".json"; }
+
+std::string read_file(const std::string& path) {
+    std::ifstream f(path);
+    EXPECT_TRUE(f.good());
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+std::vector<PerfResult> one_result(const char* label, std::size_t n, double ms) {
+    PerfResult r;
+    r.label = label;
+    r.grid_size = n;
+    r.time_ms = ms;
+    return {r};
+}
+
+}  // namespace
+
+TEST(WriteResultsJson, RecordsTileConfigFromTheLaunchShape) {
+    const std::string path = io_path();
+    write_results_json(path, one_result("naive", 512, 11.5), 0, 200, 20);
+    const std::string content = read_file(path);
+    stencil_grid g{512, 512, 512, 1.0, 1.0, 1.0, 1.0};
+    stencil_tile_shape sh{};
+    EXPECT_GE(stencil_tuned_shape(&g, 0, &sh), 0);
+    const int chunk = ((504 + sh.n_chunks - 1) / sh.n_chunks + 8) / 9 * 9;
+    EXPECT_NE(content.find("\"tile_x\": " + std::to_string(4 * sh.tile_x4)), std::string::npos);
+    EXPECT_NE(content.find("\"tile_y\": " + std::to_string(sh.tile_y)), std::string::npos);
+    EXPECT_NE(content.find("\"pencil_z\": " + std::to_string(chunk < 504 ? chunk : 504)), std::string::npos);
+    EXPECT_NE(content.find("\"stencil_radius\": 4"), std::string::npos);
+    EXPECT_NE(content.find("\"precision\": \"fp32\""), std::string::npos);
+    std::remove(path.c_str());
+}
+
+TEST(WriteResultsJson, RecordsStepsAndWarmupFromArguments) {
+    const std::string path = io_path();
+    write_results_json(path, one_result("tiled", 256, 2.0), 0, 200, 20);
+    const std::string content = read_file(path);
+    EXPECT_NE(content.find("\"steps\": 200"), std::string::npos);
+    EXPECT_NE(content.find("\"warmup\": 20"), std::string::npos);
+    std::remove(path.c_str());
+}
+
+TEST(WriteResultsJson, DifferentStepsWarmupProduceDifferentOutput) {
+    const std::string path = io_path();
+    const auto results = one_result("naive", 64, 0.1);
+    write_results_json(path, results, 0, 50, 5);
+    const std::string a = read_file(path);
+    EXPECT_NE(a.find("\"steps\": 50"), std::string::npos);
+    EXPECT_NE(a.find("\"warmup\": 5"), std::string::npos);
+    write_results_json(path, results, 0, 999, 99);
+    const std::string b = read_file(path);
+    EXPECT_NE(b.find("\"steps\": 999"), std::string::npos);
+    EXPECT_NE(b.find("\"warmup\": 99"), std::string::npos);
+    EXPECT_EQ(b.find("\"steps\": 50"), std::string::npos);  // overwritten, not appended
+    std::remove(path.c_str());
+}
+
+TEST(WriteResultsJson, EmptyPathIsNoOp) {
+    EXPECT_NO_THROW(write_results_json("", one_result("naive", 64, 0.1), 0, 10, 1));
+}
+
+TEST(WriteResultsJson, PerResultFieldsStillPresent) {
+    const std::string path = io_path();
+    std::vector<PerfResult> results = one_result("tiled", 384, 6.743);
+    results[0].bandwidth_gbs = 882.9;
+    results[0].gflops = 441.4;
+    write_results_json(path, results, 0, 200, 20);
+    const std::string content = read_file(path);
+    EXPECT_NE(content.find("\"label\": \"tiled\""), std::string::npos);
+    EXPECT_NE(content.find("\"grid_size\": 384"), std::string::npos);
+    EXPECT_NE(content.find("6.743"), std::string::npos);
+    EXPECT_NE(content.find("882.9"), std::string::npos);
+    EXPECT_NE(content.find("441.4"), std::string::npos);
+    std::remove(path.c_str());
+}
+
+TEST(WriteScalingJson, RecordsEveryPoint) {
+    const std::string path = io_path();
+    std::vector<ScalingResult> rs(2);
+    rs[0].mode = "weak";
+    rs[0].ngpu = 1;
+    rs[0].time_ms = 10.5;
+    rs[0].efficiency = 1.0;
+    rs[1].mode = "strong";
+    rs[1].ngpu = 8;
+    rs[1].time_ms = 1.75;
+    rs[1].efficiency = 0.75;
+    write_scaling_json(path, rs);
+    const std::string content = read_file(path);
+    EXPECT_NE(content.find("\"mode\": \"weak\""), std::string::npos);
+    EXPECT_NE(content.find("\"mode\": \"strong\""), std::string::npos);
+    EXPECT_NE(content.find("\"ngpu\": 8"), std::string::npos);
+    EXPECT_NE(content.find("1.75"), std::string::npos);
+    EXPECT_NE(content.find("0.75"), std::string::npos);
+    std::remove(path.c_str());
+}
