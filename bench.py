@@ -394,9 +394,14 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
                      "energy_drift": (eb - ea) / ea}
         del diff
 
+    if tune:  # FP32 gets the same tuning as the 16-bit formats (cached on repeat sizes)
+        tb = [p0.clone(), p0.clone(), torch.zeros_like(p0)]
+        ss.tune_tiled(tb[1], tb[0], vel2, tb[2], grid, reps=3)
+        del tb
     last, ms, ea, eb = ring(lambda c, p, o: ss.launch_tiled(c, p, vel2, o, grid), lambda: p0.clone(),
                             lambda: torch.zeros_like(p0), v_for_energy=vel2)
     report("fp32", last, ms, ea, eb, 16.0)
+    out["fp32"]["tile_shape"] = ss.tuned_shape(grid, "wave")[0]
     del last
     for fmt in ("bf16", "fp16", "bfp16"):
         vs = ss.pow2_scale_for(float(vel2.max())) if fmt == "fp16" else 1.0
