@@ -255,7 +255,7 @@ int stencil_launch_tiled_16(int storage, const stencil_field16* p_cur, const ste
         return STENCIL_ERR_INVALID;
     if (!pow2(p_scale) || !pow2(v_scale)) return STENCIL_ERR_INVALID;
     const Dims d = dims_of(grid);
-    if (d.nx % 16 != 0 || d.nx * d.ny * 2 >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
+    if (d.nx % 16 != 0 || d.nx * d.ny * 4 * 30 >= (int64_t(1) << 32)) return STENCIL_ERR_UNSUPPORTED;
     ZqShape shape;
     if (!shape_of(tile, d, 2 + storage, &shape)) return STENCIL_ERR_INVALID;
     return hip_status(wave_zq_16(storage, as_field(p_cur), as_field(p_prev), as_field(vel2),

@@ -41,7 +41,7 @@ struct ZqShape {
 };
 
 // Whether the kernel can take these dims and pointers (nx % 4 == 0, 16-byte
-// aligned bases, in-plane offsets below 2^32).
+// aligned bases, 30 planes below 2^32 bytes: nx * ny < 35.7 M cells).
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
 // Whether (txv, ty) is a tile shape the library is built with.
 bool zq_shape_valid(int txv, int ty);

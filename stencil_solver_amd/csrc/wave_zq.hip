@@ -101,30 +101,40 @@ using dev::Fmt;
 template <int FMT>
 using Raw = typename std::conditional<FMT == dev::kF32, float4, uint2>::type;
 
-// Every field access goes through a raw buffer resource for one z-plane
-// (base = the plane's first byte, built in SGPRs; range = the plane's bytes):
-// plane addressing is scalar work, each lane supplies a 32-bit byte offset,
-// and an offset outside the plane reads 0 / drops the write instead of
-// touching other memory.  0x00020000 is the gfx9 word 3 (32-bit data format).
+// Every field access goes through a raw buffer resource (built once per
+// workgroup in SGPRs) whose base is the first z-plane the workgroup touches and
+// whose range covers its planes; an access names its plane with the
+// instruction's scalar offset, (plane - first plane) * plane bytes, and its
+// place in the plane with a 32-bit per-lane byte offset.  Plane addressing is
+// one scalar multiply per plane and field.  0x00020000 is the gfx9 word 3
+// (32-bit data format).
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 __device__ __forceinline__ Rsrc plane_rsrc(const void* base, int z, size_t bytes) {
     char* p = const_cast<char*>(static_cast<const char*>(base)) + static_cast<size_t>(z) * bytes;
     return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(static_cast<unsigned>(bytes)), 0x00020000);
 }
+// planes first .. first + n_planes - 1 of a field with plane_bytes per plane
+// (n_planes * plane_bytes < 2^32: the launcher bounds the z-chunk length).
+__device__ __forceinline__ Rsrc span_rsrc(const void* base, int first, size_t plane_bytes, unsigned n_planes) {
+    char* p = const_cast<char*>(static_cast<const char*>(base)) + static_cast<size_t>(first) * plane_bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, static_cast<int>(static_cast<unsigned>(plane_bytes) * n_planes),
+                                             0x00020000);
+}
 // Cache policy bits of gfx950 buffer accesses: 2 = non-temporal.
 constexpr int kNtAux = 2;
 
-// Loads the 4 stored values at byte offset boff of a plane.
+// Loads the 4 stored values at byte offset boff of the plane at scalar
+// offset soff.
 template <int FMT, bool NT>
-__device__ __forceinline__ Raw<FMT> load_raw(Rsrc r, unsigned boff) {
+__device__ __forceinline__ Raw<FMT> load_raw(Rsrc r, unsigned boff, int soff) {
     constexpr int aux = NT ? kNtAux : 0;
     if constexpr (FMT == dev::kF32) {
         typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(boff), 0, aux);
+        const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(boff), soff, aux);
         return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                            __uint_as_float(v.w));
     } else {
-        const dev::v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(boff), 0, aux);
+        const dev::v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(boff), soff, aux);
         return make_uint2(v.x, v.y);
     }
 }
@@ -147,15 +157,15 @@ __device__ __forceinline__ float4 decode(Raw<FMT> u, int e, float inv) {
 // `blk` whether its quad's block holds outputs (then lane 0 of the quad writes
 // the scale at byte eoff of the plane's scale row `er`).
 template <int FMT, bool NT, bool SCALED>
-__device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned eoff, float4 v,
-                                       float scale, bool vals, bool blk) {
+__device__ __forceinline__ void store4(Rsrc r, int soff, Rsrc er, int esoff, unsigned boff, unsigned eoff,
+                                       float4 v, float scale, bool vals, bool blk) {
     constexpr int aux = NT ? kNtAux : 0;
     if constexpr (FMT == dev::kF32) {
         if (vals) {
             typedef unsigned v4u __attribute__((ext_vector_type(4)));
             const v4u w = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
                            __float_as_uint(v.w)};
-            __builtin_amdgcn_raw_buffer_store_b128(w, r, static_cast<int>(boff), 0, aux);
+            __builtin_amdgcn_raw_buffer_store_b128(w, r, static_cast<int>(boff), soff, aux);
         }
     } else {
         uint2 u;
@@ -171,11 +181,12 @@ __device__ __forceinline__ void store4(Rsrc r, Rsrc er, unsigned boff, unsigned 
         }
         if (vals) {
             const dev::v2u w = {u.x, u.y};
-            __builtin_amdgcn_raw_buffer_store_b64(w, r, static_cast<int>(boff), 0, aux);
+            __builtin_amdgcn_raw_buffer_store_b64(w, r, static_cast<int>(boff), soff, aux);
         }
         if constexpr (FMT == dev::kBfp16)
             if (blk && (threadIdx.x & 3) == 0)
-                __builtin_amdgcn_raw_buffer_store_b8(static_cast<unsigned char>(e), er, static_cast<int>(eoff), 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b8(static_cast<unsigned char>(e), er, static_cast<int>(eoff), esoff,
+                                                     0);
     }
 }
 
@@ -303,10 +314,6 @@ void zq_kernel(ZqArgs a) {
                            : 0u;
     }
 
-    const char* const cur_b = static_cast<const char*>(a.cur);
-    const char* const prev_b = static_cast<const char*>(a.prev);
-    const char* const vel_b = static_cast<const char*>(a.vel2);
-    char* const next_b = static_cast<char*>(a.next);
     const float p_inv = a.p_inv, v_inv = a.v_inv;
 
     // Loads are unconditional.  A lane whose cell lies outside the grid (x or y
@@ -333,26 +340,46 @@ void zq_kernel(ZqArgs a) {
         h_eld[k] = h_in[k] ? (h_off[k] >> 4) : 0u;
     }
     const int z_last = a.nz - 1;
-    // Kernel plane z -> memory plane (uniform: scalar arithmetic).
+    // Kernel planes this workgroup touches: z_begin-4 (queue fill) up to the
+    // last prefetch of its last group of 9 steps (z_end+14 at most), clamped
+    // to the grid; the kernel's plane z is memory plane z_org + z_sgn * z.
     const int z_sgn = a.mirror ? -1 : 1;
     const int z_org = a.mirror ? z_last : 0;
-    auto prs = [z_sgn, z_org](const void* base, int z, size_t bytes) {
-        return plane_rsrc(base, z_org + z_sgn * z, bytes);
+    const int zl_lo = z_begin - kR;
+    const int zl_hi = min(z_end + 16, z_last);
+    const int p_lo = a.mirror ? z_last - zl_hi : zl_lo;  // first memory plane
+    const unsigned n_span = static_cast<unsigned>(zl_hi - zl_lo + 1);
+    // scalar offset of kernel plane z in a field with pb bytes per plane
+    auto soff = [z_sgn, z_org, p_lo](int z, size_t pb) -> int {
+        return static_cast<int>(static_cast<unsigned>(z_org + z_sgn * z - p_lo) * static_cast<unsigned>(pb));
     };
-    auto raw_at = [plane_bytes, prs](const char* f, int z, unsigned boff) -> Raw<FMT> {
-        return load_raw<FMT, false>(prs(f, z, plane_bytes), boff);
+    auto field = [p_lo, n_span](const void* base, size_t pb) { return span_rsrc(base, p_lo, pb, n_span); };
+    auto raw_at = [plane_bytes, soff](Rsrc f, int z, unsigned boff) -> Raw<FMT> {
+        return load_raw<FMT, false>(f, boff, soff(z, plane_bytes));
     };
-    auto raw_stream = [plane_bytes, own_ld, prs](const char* f, int z) -> Raw<FMT> {
-        return load_raw<FMT, NT>(prs(f, z, plane_bytes), own_ld);
+    auto raw_stream = [plane_bytes, own_ld, soff](Rsrc f, int z) -> Raw<FMT> {
+        return load_raw<FMT, NT>(f, own_ld, soff(z, plane_bytes));
     };
-    auto exp_at = [eplane, prs](const int8_t* e, int z, unsigned eoff) -> int {
+    auto exp_at = [eplane, soff](Rsrc e, int z, unsigned eoff) -> int {
         (void)eplane;
-        (void)prs;
+        (void)soff;
         if constexpr (kBlk)
             return static_cast<signed char>(
-                __builtin_amdgcn_raw_buffer_load_b8(prs(e, z, eplane), static_cast<int>(eoff), 0, 0));
+                __builtin_amdgcn_raw_buffer_load_b8(e, static_cast<int>(eoff), soff(z, eplane), 0));
         return 0;
     };
+    // The fields this mode touches (unused ones are dropped by the compiler).
+    const Rsrc cur_r = field(a.cur, plane_bytes);
+    const Rsrc prev_r = field(a.prev, plane_bytes);
+    const Rsrc vel_r = field(a.vel2, plane_bytes);
+    const Rsrc next_r = field(a.next, plane_bytes);
+    const Rsrc aux_r = field(a.aux, plane_bytes);
+    const Rsrc grad_r = field(a.grad, plane_bytes);
+    const Rsrc lap_r = field(a.lap_out, plane_bytes);
+    const Rsrc cur_er = field(a.cur_e, eplane);
+    const Rsrc prev_er = field(a.prev_e, eplane);
+    const Rsrc vel_er = field(a.vel2_e, eplane);
+    const Rsrc next_er = field(a.next_e, eplane);
     // This lane's quad (16 x-consecutive cells) holds output cells: it owns
     // the block exponent of kBfp16 storage.
     const int xq = x & ~15;
@@ -450,11 +477,11 @@ void zq_kernel(ZqArgs a) {
                     if (at_src && a.src_val) dev::set_comp(out, a.src_x - x, comp(out, a.src_x - x) + *a.src_val);
                     // receiver plane: p^{n+1} on plane rec_z into the trace (its halo is pre-zeroed)
                     if (a.trace && zz == a.rec_z)
-                        store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), plane_rsrc(a.trace, 0, 0),
-                                                 own_b, 0, out, 1.f, own_out, false);
+                        store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), 0, plane_rsrc(a.trace, 0, 0),
+                                                 0, own_b, 0, out, 1.f, own_out, false);
                     if (a.lap_out)
-                        store4<FMT, NT, true>(prs(a.lap_out, zz, plane_bytes), prs(a.lap_out, zz, 0),
-                                              own_b, 0, lap4, 1.f, own_out, false);
+                        store4<FMT, NT, true>(lap_r, soff(zz, plane_bytes), lap_r, 0, own_b, 0, lap4, 1.f,
+                                              own_out, false);
                 } else {
                     if (a.res && zz == a.rec_z && own_out) {  // + m * residual
                         const float4 r = *reinterpret_cast<const float4*>(a.res + own_off);
@@ -465,29 +492,27 @@ void zq_kernel(ZqArgs a) {
                         const float4 c = zq[kR];
                         const float4 g = make_float4(gr.x + a.s.dt2 * lc.x * c.x, gr.y + a.s.dt2 * lc.y * c.y,
                                                      gr.z + a.s.dt2 * lc.z * c.z, gr.w + a.s.dt2 * lc.w * c.w);
-                        store4<FMT, NT, true>(prs(a.grad, zz, plane_bytes), prs(a.grad, zz, 0),
-                                              own_b, 0, g, 1.f, own_out, false);
+                        store4<FMT, NT, true>(grad_r, soff(zz, plane_bytes), grad_r, 0, own_b, 0, g, 1.f,
+                                              own_out, false);
                     }
                     if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x) / comp(vv, a.src_x - x);
                 }
             }
-            store4<FMT, NT, kExact>(prs(next_b, zz, plane_bytes),
-                                    prs(a.next_e, zz, kBlk ? eplane : 0), own_b, own_off >> 4, out,
-                                    a.p_scale, own_out, blk_out);
+            store4<FMT, NT, kExact>(next_r, soff(zz, plane_bytes), next_er, kBlk ? soff(zz, eplane) : 0, own_b,
+                                    own_off >> 4, out, a.p_scale, own_out, blk_out);
         }
     };
 
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
-    const char* const aux_b = static_cast<const char*>(a.aux);
     float4 q[9];
     if constexpr (FMT == dev::kF32) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = raw_at(cur_b, z_begin - kR + k, own_ld);
+        for (int k = 0; k < 8; ++k) q[k] = raw_at(cur_r, z_begin - kR + k, own_ld);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            q[k] = decode<FMT, kExact>(raw_at(cur_b, z_begin - kR + k, own_ld),
-                                       exp_at(a.cur_e, z_begin - kR + k, own_eld), p_inv);
+            q[k] = decode<FMT, kExact>(raw_at(cur_r, z_begin - kR + k, own_ld),
+                                       exp_at(cur_er, z_begin - kR + k, own_eld), p_inv);
     }
 
     if constexpr (FMT == dev::kF32) {
@@ -504,20 +529,20 @@ void zq_kernel(ZqArgs a) {
         // Loads of plane z into FIFO stage d (queue refill: plane z + kR).
         auto fetch = [&](int z, float4& r, float4& xe, float4 (&h)[kHaloPer], float4& pv, float4& vv, float4& lc,
                          float4& gr) {
-            r = raw_at(cur_b, min(z + kR, z_last), own_ld);
+            r = raw_at(cur_r, min(z + kR, z_last), own_ld);
             xe = zero4;
-            if constexpr (kShuffleX) xe = raw_at(cur_b, z, xe_ld);
+            if constexpr (kShuffleX) xe = raw_at(cur_r, z, xe_ld);
 #pragma unroll
-            for (int k = 0; k < kHaloPer; ++k) h[k] = raw_at(cur_b, z, h_ld[k]);
+            for (int k = 0; k < kHaloPer; ++k) h[k] = raw_at(cur_r, z, h_ld[k]);
             pv = vv = lc = gr = zero4;
             if constexpr (!kHeat) {
-                pv = raw_stream(prev_b, z);
-                vv = raw_stream(vel_b, z);
+                pv = raw_stream(prev_r, z);
+                vv = raw_stream(vel_r, z);
             }
             if constexpr (kAdj) {
                 if (lap_in) {
-                    lc = raw_stream(aux_b, z);
-                    gr = raw_stream(static_cast<const char*>(a.grad), z);
+                    lc = raw_stream(aux_r, z);
+                    gr = raw_stream(grad_r, z);
                 }
             }
         };
@@ -577,18 +602,18 @@ void zq_kernel(ZqArgs a) {
         bool h_use[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) h_use[k] = tid + k * kThreads < kHalo;
-        Raw<FMT> rq = raw_at(cur_b, z_begin + kR, own_ld);
-        int rq_e = exp_at(a.cur_e, z_begin + kR, own_eld);
+        Raw<FMT> rq = raw_at(cur_r, z_begin + kR, own_ld);
+        int rq_e = exp_at(cur_er, z_begin + kR, own_eld);
         Raw<FMT> rh[kHaloPer];
         int rh_e[kHaloPer];
 #pragma unroll
         for (int k = 0; k < kHaloPer; ++k) {
-            rh[k] = raw_at(cur_b, z_begin, h_ld[k]);
-            rh_e[k] = exp_at(a.cur_e, z_begin, h_eld[k]);
+            rh[k] = raw_at(cur_r, z_begin, h_ld[k]);
+            rh_e[k] = exp_at(cur_er, z_begin, h_eld[k]);
         }
-        Raw<FMT> rp = raw_stream(prev_b, z_begin), rv = raw_stream(vel_b, z_begin);
-        int rp_e = exp_at(a.prev_e, z_begin, own_eld);
-        int rv_e = exp_at(a.vel2_e, z_begin, own_eld);
+        Raw<FMT> rp = raw_stream(prev_r, z_begin), rv = raw_stream(vel_r, z_begin);
+        int rp_e = exp_at(prev_er, z_begin, own_eld);
+        int rv_e = exp_at(vel_er, z_begin, own_eld);
         for (int zb = z_begin; zb < z_end; zb += 9) {
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
@@ -610,17 +635,17 @@ void zq_kernel(ZqArgs a) {
                 } else {
                     vv = decode<FMT, kExact>(rv, rv_e, v_inv);
                 }
-                rq = raw_at(cur_b, z5, own_ld);
-                rq_e = exp_at(a.cur_e, z5, own_eld);
+                rq = raw_at(cur_r, z5, own_ld);
+                rq_e = exp_at(cur_er, z5, own_eld);
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k) {
-                    rh[k] = raw_at(cur_b, z1, h_ld[k]);
-                    rh_e[k] = exp_at(a.cur_e, z1, h_eld[k]);
+                    rh[k] = raw_at(cur_r, z1, h_ld[k]);
+                    rh_e[k] = exp_at(cur_er, z1, h_eld[k]);
                 }
-                rp = raw_stream(prev_b, z1);
-                rv = raw_stream(vel_b, z1);
-                rp_e = exp_at(a.prev_e, z1, own_eld);
-                rv_e = exp_at(a.vel2_e, z1, own_eld);
+                rp = raw_stream(prev_r, z1);
+                rv = raw_stream(vel_r, z1);
+                rp_e = exp_at(prev_er, z1, own_eld);
+                rv_e = exp_at(vel_er, z1, own_eld);
                 const int b = (zz - z_begin) & 1;
                 lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
@@ -759,6 +784,11 @@ hipError_t launch_zq(ZqArgs a, const Dims& d, ZqShape shape, int mode, hipStream
     a.tiles_x = static_cast<int>((d.nx + 4 * shape.txv - 1) / (4 * shape.txv));
     a.tiles_y = static_cast<int>((d.ny - 2 * kR + shape.ty - 1) / shape.ty);
     a.chunk = zq_chunk_planes(inz, shape.n_chunks);
+    // A workgroup's buffer resources span its chunk plus 21 planes (queue fill
+    // and prefetch); keep that below 2^32 bytes with shorter chunks if needed.
+    const int64_t max_chunk = (int64_t(0xFFFFFFFF) / (d.nx * d.ny * 4) - 21) / 9 * 9;
+    if (max_chunk < 9) return hipErrorInvalidValue;  // zq_supported() excludes such planes
+    if (a.chunk > max_chunk) a.chunk = static_cast<int>(max_chunk);
     const int64_t n_chunks = (inz + a.chunk - 1) / a.chunk;
     const int64_t nwg = int64_t(a.tiles_x) * a.tiles_y * n_chunks;
     a.xcd_remap = shape.xcd_remap != 0 ? 1 : 0;
@@ -787,8 +817,9 @@ int zq_chunk_planes(int64_t interior_planes, int n_chunks) {
 
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs) {
     if (d.nx % 4 != 0) return false;
-    // 32-bit in-plane byte offsets.
-    if (d.nx * d.ny * 4 >= (int64_t(1) << 32)) return false;
+    // 32-bit byte offsets within a workgroup's planes: at least 30 planes
+    // (one 9-plane chunk + 21) must fit in 2^32 bytes.
+    if (d.nx * d.ny * 4 * 30 >= (int64_t(1) << 32)) return false;
     for (int i = 0; i < n_ptrs; ++i)
         if (reinterpret_cast<uintptr_t>(ptrs[i]) % 16 != 0) return false;
     return true;
