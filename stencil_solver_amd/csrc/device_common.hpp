@@ -149,8 +149,9 @@ __device__ __forceinline__ float max_abs4(float4 v) {
 // m >= 0, so the int ordering of the bits is the float ordering.
 __device__ __forceinline__ float quad_max(float m) {
     int v = __float_as_int(m);
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));
+    // quad_perm moves always have a source lane, so no old value is needed
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
     return __int_as_float(v);
 }
 

@@ -151,11 +151,11 @@ __device__ __forceinline__ float4 decode(Raw<FMT> u, int e, float inv) {
     else return dev::bfp16x4_to_float4(u, e);
 }
 
-// Stores 4 values at byte offset boff of a plane.  Every lane of a wave
-// reaches this (kBfp16 shares one block scale across a quad of lanes); `vals`
-// says whether this lane writes its values (v is zero where it does not),
-// `blk` whether its quad's block holds outputs (then lane 0 of the quad writes
-// the scale at byte eoff of the plane's scale row `er`).
+// Stores 4 values at byte offset boff of the plane at scalar offset soff.
+// Every lane of a wave reaches this (kBfp16 shares one block scale across a
+// quad of lanes); `vals` says whether this lane writes its values, `blk`
+// whether its quad's block holds outputs (then lane 0 of the quad writes the
+// scale at byte eoff of the scale plane at esoff of `er`).
 template <int FMT, bool NT, bool SCALED>
 __device__ __forceinline__ void store4(Rsrc r, int soff, Rsrc er, int esoff, unsigned boff, unsigned eoff,
                                        float4 v, float scale, bool vals, bool blk) {
@@ -171,8 +171,8 @@ __device__ __forceinline__ void store4(Rsrc r, int soff, Rsrc er, int esoff, uns
         uint2 u;
         int e = 0;
         if constexpr (FMT == dev::kBfp16) {
-            // v is zero in lanes that store nothing, so they do not raise the max
-            e = dev::bfp16_exponent(dev::quad_max(dev::max_abs4(v)));
+            // lanes that store nothing do not raise the block maximum
+            e = dev::bfp16_exponent(dev::quad_max(vals ? dev::max_abs4(v) : 0.f));
             u = dev::float4_to_bfp16x4(v, e);
         } else if constexpr (FMT == dev::kBf16) {
             u = dev::float4_to_bf16x4(v);
@@ -464,11 +464,13 @@ void zq_kernel(ZqArgs a) {
     // is not a multiple of 9) run on clamped planes and store nothing.
     auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 edge, float4 pv, float4 vv, float dt2v,
                       float4 lc, float4 gr) {
-        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 lap4 = out;
+        // Every lane computes (its inputs are loaded, from offset 0 outside
+        // the grid); only output lanes store, and block-FP16 leaves the other
+        // lanes out of its block maximum.
+        float4 lap4 = make_float4(0.f, 0.f, 0.f, 0.f);
         float4 xl, xr;
         x_neighbours(b, zq[kR], edge, xl, xr);
-        if (own_out) out = compute(b, zq, xl, xr, pv, vv, dt2v, lc, lap4);
+        float4 out = compute(b, zq, xl, xr, pv, vv, dt2v, lc, lap4);
         if (zz < z_end) {
             if constexpr (kFwd || kAdj) {
                 // the source / trace cell lies in this lane's 4 cells of this plane
