@@ -144,8 +144,8 @@ int stencil_tune_tiled_f32(const float* p_cur, const float* p_prev, const float*
 int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* grid, int reps,
                           stencil_tile_shape* best, const char* csv_path, void* hip_stream);
 /* The shape a NULL-shape launch would use for these dims (kind 0 wave, 1
- * heat, 2 + storage for the 16-bit wave step): returns 1 if it came from the
- * tuner cache, 0 for the default. */
+ * heat, 2 + storage for the 16-bit wave step, 6 FWI forward step, 7 adjoint
+ * step): returns 1 if it came from the tuner cache, 0 for the default. */
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
 void stencil_clear_tuning(void);
 
@@ -278,6 +278,17 @@ int stencil_fwi_backward_f32(const float* vel2, const stencil_grid* grid, const 
 int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const stencil_fwi_config* cfg,
                              const double* wavelet, double* grad, double* src_trace, void* work,
                              void* hip_stream);
+/* Autotuner of the FP32 FWI kernels (no reference counterpart): the sweep of
+ * stencil_tune_tiled_f32 applied to the forward step, timed in both of its
+ * forms (recording the receiver plane; writing L p), and to the adjoint step,
+ * non-temporal streams only.  Runs on the workspace's wavefield and adjoint
+ * buffers; the checkpoints, traces and stored L p of an earlier forward pass
+ * stay intact.  The fastest shapes are cached for this grid
+ * (stencil_tuned_shape kinds 6 and 7) and used by the tiled FWI calls on it.
+ * CSV rows: kernel, grid, shape, ms per launch.  Synchronises. */
+int stencil_tune_fwi_f32(const stencil_grid* grid, const stencil_fwi_config* cfg, void* work, int reps,
+                         stencil_tile_shape* best_forward, stencil_tile_shape* best_adjoint,
+                         const char* csv_path, void* hip_stream);
 /* The single steps: the acoustic step (reference association) plus
  * wavelet[n] at linear cell src_index (< 0: none), writing L p_cur into
  * lap_out (nullable); and the adjoint step in the variable psi = vel2 *

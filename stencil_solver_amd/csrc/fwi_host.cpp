@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <vector>
 
 #include "launchers.hpp"
 #include "stencil_capi.h"
@@ -59,7 +61,8 @@ struct StepCtx {
     long long src = -1;  // linear cell, -1 = none
     int src_z = -1, src_y = -1, src_x = -1;
     bool tiled = false;  // FP32 on the LDS z-queue kernel
-    ZqShape shape{};
+    ZqShape shape{};      // forward steps
+    ZqShape adj_shape{};  // adjoint steps
 };
 
 StepCtx step_ctx(const Dims& d, long long src) {
@@ -72,8 +75,10 @@ StepCtx step_ctx(const Dims& d, long long src) {
         c.src_y = int((src % plane) / d.nx);
         c.src_x = int(src % d.nx);
     }
+    // The FWI tuner's shapes, else the wave step's, else defaults.
     ZqShape tuned;
-    if (zq_cached_shape(d, 0, &tuned)) c.shape = tuned;
+    if (zq_cached_shape(d, kKindFwiForward, &tuned) || zq_cached_shape(d, 0, &tuned)) c.shape = tuned;
+    if (zq_cached_shape(d, kKindFwiAdjoint, &tuned) || zq_cached_shape(d, 0, &tuned)) c.adj_shape = tuned;
     return c;
 }
 
@@ -153,8 +158,8 @@ struct Ops<float> {
         const void* ptrs[4] = {l1, l2, m, lo};
         if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || (aligned16(lap) && aligned16(g))) &&
             (!res || aligned16(res)))
-            return adjoint_zq_f32(l1, l2, m, lap, g, lo, res, rz, c.src_z, c.src_y, c.src_x, tr, c.d, s, c.shape,
-                                  st);
+            return adjoint_zq_f32(l1, l2, m, lap, g, lo, res, rz, c.src_z, c.src_y, c.src_x, tr, c.d, s,
+                                  c.adj_shape, st);
         return fwi_adj_step_f32(l1, l2, m, lap, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
     }
     static hipError_t residual(const float* a, const float* b, float* r, size_t n, double* part, double* out,
@@ -272,6 +277,10 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     for (auto* l : lam) FWI_TRY(Ops<T>::fill(l, p.cells, st));
     int l1 = 0, l2 = 1, lo = 2;  // psi^{k+1}, psi^{k+2}, psi^k (psi = m lambda)
     const bool lap_stored = p.C == p.n;  // the misfit forward wrote every L p^k
+    // The checkpoint copies restore two levels whole; the steps write only
+    // interiors, so the third level's halo (the zero walls) is cleared here
+    // rather than left to whatever last used the buffer.
+    if (!lap_stored) FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
     for (int64_t sg = p.S - 1; sg >= 0; --sg) {
         const int64_t k_lo = sg * p.C + 1;
         const int64_t k_hi = std::min((sg + 1) * p.C, p.n);
@@ -309,6 +318,96 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     return STENCIL_OK;
 }
 
+void put_shape(const ZqShape& z, stencil_tile_shape* out) {
+    if (!out) return;
+    out->tile_x4 = z.txv;
+    out->tile_y = z.ty;
+    out->n_chunks = z.n_chunks;
+    out->xcd_remap = z.xcd_remap;
+    out->nontemporal = z.nt;
+}
+
+// Launch-shape sweep of the two FP32 FWI kernels on the workspace's wavefield
+// and adjoint buffers: the forward step in both of its forms (recording the
+// receiver plane, as the forward pass runs it; writing L p, as the segment
+// recompute does) and the adjoint step, over the tuner's shapes with
+// non-temporal streams, the only mode these kernels run in.
+int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int reps, stencil_tile_shape* best_fwd,
+         stencil_tile_shape* best_adj, const char* csv_path, hipStream_t st) {
+    Plan p;
+    if (!work || reps <= 0 || !make_plan(g, c, sizeof(float), &p)) return STENCIL_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
+    const size_t field = up(p.cells * sizeof(float));
+    float* m = at<float>(work, p.ring);
+    float* f1 = at<float>(work, p.ring + field);
+    float* f2 = at<float>(work, p.ring + 2 * field);
+    float* o0 = at<float>(work, p.lam);
+    float* o1 = at<float>(work, p.lam + field);
+    float* o2 = at<float>(work, p.lam + 2 * field);
+    float* trace = at<float>(work, p.partial);  // reduction scratch
+    const void* ptrs[4] = {m, f1, f2, o0};
+    if (!zq_supported(p.d, ptrs, 4)) return STENCIL_ERR_UNSUPPORTED;
+    // (2000 m/s)^2 and fields of ones
+    FWI_TRY(fill_f32(m, p.cells, 4.0e6f, st));
+    for (float* f : {f1, f2, o0, o1, o2}) FWI_TRY(fill_f32(f, p.cells, 1.0f, st));
+    const Scalars<float> s = scalars<float>(g);
+    const StepCtx cx = step_ctx(p.d, p.src);
+    struct Row {
+        ZqShape shape;
+        float fwd_ms, adj_ms;
+    };
+    std::vector<Row> rows;
+    for (const ZqShape& sh : zq_tune_candidates(p.d)) {
+        if (sh.nt != 1) continue;
+        Row r{sh, -1.f, -1.f};
+        const float pair = zq_time_launches(
+            [&] {
+                const hipError_t e = wave_zq_fwi_f32(f1, f2, m, o0, nullptr, cx.src_z, cx.src_y, cx.src_x, f1, o2,
+                                                     p.rec_z, p.d, s, sh, st);
+                if (e != hipSuccess) return e;
+                return wave_zq_fwi_f32(f1, f2, m, o0, o1, cx.src_z, cx.src_y, cx.src_x, f1, nullptr, 0, p.d, s, sh,
+                                       st);
+            },
+            reps, st);
+        if (pair > 0.f) r.fwd_ms = 0.5f * pair;
+        // the adjoint runs 64 x 4 and 64 x 16 as 64 x 8: not timed twice
+        if (!(sh.txv == 64 && (sh.ty == 4 || sh.ty == 16)))
+            r.adj_ms = zq_time_launches(
+                [&] {
+                    return adjoint_zq_f32(f1, f2, m, o1, o2, o0, f1, p.rec_z, cx.src_z, cx.src_y, cx.src_x, trace,
+                                          p.d, s, sh, st);
+                },
+                reps, st);
+        rows.push_back(r);
+    }
+    const Row* bf = nullptr;
+    const Row* ba = nullptr;
+    for (const Row& r : rows) {
+        if (r.fwd_ms > 0.f && (!bf || r.fwd_ms < bf->fwd_ms)) bf = &r;
+        if (r.adj_ms > 0.f && (!ba || r.adj_ms < ba->adj_ms)) ba = &r;
+    }
+    if (!bf || !ba) return STENCIL_ERR_HIP;
+    zq_cache_put(p.d, kKindFwiForward, bf->shape);
+    zq_cache_put(p.d, kKindFwiAdjoint, ba->shape);
+    put_shape(bf->shape, best_fwd);
+    put_shape(ba->shape, best_adj);
+    if (csv_path && *csv_path) {
+        FILE* f = std::fopen(csv_path, "w");
+        if (!f) return STENCIL_ERR_INVALID;
+        std::fprintf(f, "kernel,nx,ny,nz,tile_x,tile_y,n_chunks,xcd_remap,ms\n");
+        for (const Row& r : rows)
+            for (int k = 0; k < 2; ++k) {
+                const float ms = k == 0 ? r.fwd_ms : r.adj_ms;
+                if (ms > 0.f)
+                    std::fprintf(f, "%s,%lld,%lld,%lld,%d,%d,%d,%d,%.5f\n", k == 0 ? "forward" : "adjoint",
+                                 (long long)p.d.nx, (long long)p.d.ny, (long long)p.d.nz, 4 * r.shape.txv,
+                                 r.shape.ty, r.shape.n_chunks, r.shape.xcd_remap, ms);
+            }
+        std::fclose(f);
+    }
+    return STENCIL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -341,6 +440,11 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
                              const double* wavelet, double* grad, double* src_trace, void* work, void* stream) {
     return backward<double>(vel2, grid, cfg, wavelet, grad, src_trace, work, static_cast<hipStream_t>(stream),
                             false);
+}
+int stencil_tune_fwi_f32(const stencil_grid* grid, const stencil_fwi_config* cfg, void* work, int reps,
+                         stencil_tile_shape* best_forward, stencil_tile_shape* best_adjoint, const char* csv_path,
+                         void* stream) {
+    return tune(grid, cfg, work, reps, best_forward, best_adjoint, csv_path, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <vector>
 
 namespace stencil::hipk {
 
@@ -45,16 +47,26 @@ struct ZqShape {
 bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
 // Whether (txv, ty) is a tile shape the library is built with.
 bool zq_shape_valid(int txv, int ty);
-// Fills unset fields of a shape.
+// Shape-cache kinds: 0 wave step, 1 heat step, 2 + storage the 16-bit wave
+// step, then the two FWI kernels.
+constexpr int kKindFwiForward = 6;
+constexpr int kKindFwiAdjoint = 7;
+// Fills unset fields of a shape for a kernel of this kind; the adjoint kind
+// also maps the tiles it is not built for (64 x 4, 64 x 16) to 64 x 8.
 ZqShape zq_resolve(const Dims& d, ZqShape s, int kind);
 // z-planes per chunk of the tiled kernel for n_chunks chunks of a column.
 int zq_chunk_planes(int64_t interior_planes, int n_chunks);
 
-// Autotuner cache (tune.cpp): the fastest shape measured for a grid size;
-// kind 0 = wave step, 1 = heat step.
+// Autotuner cache (tune.cpp): the fastest shape measured for a grid size
+// and kernel kind.
 bool zq_cached_shape(const Dims& d, int kind, ZqShape* out);
 void zq_cache_put(const Dims& d, int kind, const ZqShape& s);
 void zq_cache_clear();
+// Tuner helpers: the launch shapes swept for these dims, and the time of one
+// call of `fn` (ms per call over reps calls after 2 warmups; < 0 if a call
+// failed).
+std::vector<ZqShape> zq_tune_candidates(const Dims& d);
+float zq_time_launches(const std::function<hipError_t()>& fn, int reps, hipStream_t st);
 
 // The acoustic step.  `exact` = the reference association (bit-identical to
 // its CPU path); otherwise FMA-fused weights.

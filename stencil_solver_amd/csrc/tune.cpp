@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -48,21 +49,7 @@ void zq_cache_clear() {
     cache().clear();
 }
 
-}  // namespace stencil::hipk
-
-using namespace stencil::hipk;
-
-namespace {
-
-struct Row {
-    ZqShape shape;
-    float ms;
-};
-
-// Times `fn` (one launch) reps times after 2 warmups; returns ms per launch or
-// a negative value if a launch failed.
-template <class F>
-float time_launches(F&& fn, int reps, hipStream_t st) {
+float zq_time_launches(const std::function<hipError_t()>& fn, int reps, hipStream_t st) {
     for (int i = 0; i < 2; ++i)
         if (fn() != hipSuccess) return -1.f;
     hipEvent_t a, b;
@@ -81,7 +68,7 @@ float time_launches(F&& fn, int reps, hipStream_t st) {
     return ok ? ms / static_cast<float>(reps) : -1.f;
 }
 
-std::vector<ZqShape> candidates(const Dims& d) {
+std::vector<ZqShape> zq_tune_candidates(const Dims& d) {
     static const int shapes[][2] = {{16, 16}, {16, 32}, {32, 8}, {32, 16},
                                     {64, 4},  {64, 8},  {64, 12}, {64, 16}};
     const int64_t inz = d.nz - 8;
@@ -105,6 +92,17 @@ std::vector<ZqShape> candidates(const Dims& d) {
                 }
     return out;
 }
+
+}  // namespace stencil::hipk
+
+using namespace stencil::hipk;
+
+namespace {
+
+struct Row {
+    ZqShape shape;
+    float ms;
+};
 
 int finish(const Dims& d, int kind, const std::vector<Row>& rows, float naive_ms,
            stencil_tile_shape* best, const char* csv_path) {
@@ -158,11 +156,11 @@ int stencil_tune_tiled_f32(const float* p_cur, const float* p_prev, const float*
     const auto s = make_scalars<float>(float(grid->dx), float(grid->dy), float(grid->dz),
                                        float(grid->dt));
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    const float naive_ms = time_launches(
+    const float naive_ms = zq_time_launches(
         [&] { return wave_naive_f32(p_cur, p_prev, vel2, p_next, d, s, st); }, reps, st);
     std::vector<Row> rows;
-    for (const ZqShape& c : candidates(d)) {
-        const float ms = time_launches(
+    for (const ZqShape& c : zq_tune_candidates(d)) {
+        const float ms = zq_time_launches(
             [&] { return wave_zq_f32(p_cur, p_prev, vel2, p_next, d, s, true, c, st); }, reps, st);
         rows.push_back({c, ms});
     }
@@ -181,11 +179,11 @@ int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* gri
     const auto s = make_scalars<float>(float(grid->dx), float(grid->dy), float(grid->dz),
                                        float(grid->dt));
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    const float naive_ms = time_launches(
+    const float naive_ms = zq_time_launches(
         [&] { return heat_naive_f32(t, t_next, d, s, 0.f, st); }, reps, st);
     std::vector<Row> rows;
-    for (const ZqShape& c : candidates(d)) {
-        const float ms = time_launches(
+    for (const ZqShape& c : zq_tune_candidates(d)) {
+        const float ms = zq_time_launches(
             [&] { return heat_zq_f32(t, t_next, d, s, 0.f, true, c, st); }, reps, st);
         rows.push_back({c, ms});
     }
@@ -215,8 +213,8 @@ int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stenc
     const Field16 c{p_cur->data, p_cur->exp}, p{p_prev->data, p_prev->exp};
     const Field16 v{vel2->data, vel2->exp}, n{p_next->data, p_next->exp};
     std::vector<Row> rows;
-    for (const ZqShape& sh : candidates(d)) {
-        const float ms = time_launches(
+    for (const ZqShape& sh : zq_tune_candidates(d)) {
+        const float ms = zq_time_launches(
             [&] { return wave_zq_16(storage, c, p, v, n, d, s, p_scale, v_scale, false, sh, st); }, reps, st);
         rows.push_back({sh, ms});
     }
@@ -224,7 +222,7 @@ int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stenc
 }
 
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out) {
-    if (!dims_ok(grid) || !out || kind < 0 || kind > 2 + STENCIL_STORE_BFP16) return STENCIL_ERR_INVALID;
+    if (!dims_ok(grid) || !out || kind < 0 || kind > kKindFwiAdjoint) return STENCIL_ERR_INVALID;
     Dims d;
     d.nx = int64_t(grid->nx);
     d.ny = int64_t(grid->ny);

@@ -850,7 +850,13 @@ ZqShape zq_resolve(const Dims& d, ZqShape s, int kind) {
         }
         if (kind == 1 && best == 64) best = 32;  // heat: 128 x 16 cells (same padding)
         s.txv = best;
-        s.ty = best == 64 ? ((kind >= 2 || d.nx >= 1024) ? 16 : 8) : best == 32 ? 16 : 32;
+        s.ty = best == 64 ? ((kind >= 2 && kind <= 5) || d.nx >= 1024 ? 16 : 8) : best == 32 ? 16 : 32;
+    }
+    // 64 x 16 (1024 threads, 128 VGPRs) and 64 x 4 (three halo items per
+    // thread) spill with the adjoint's extra streams: it runs 64 x 8 instead.
+    if (kind == kKindFwiAdjoint && s.txv == 64 && (s.ty == 16 || s.ty == 4)) {
+        s.ty = 8;
+        s.n_chunks = 0;
     }
     if (s.nt < 0) s.nt = 1;
     const int64_t inz = d.nz - 2 * kR;
@@ -960,15 +966,7 @@ hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, 
     a.src_x = src_x;
     a.s = s;
     shape.nt = 1;
-    // 64 x 16 (1024 threads, 128 VGPRs) and 64 x 4 (three halo items per
-    // thread) spill with the adjoint's extra streams: use 64 x 8 instead.
-    shape = zq_resolve(d, shape, 0);
-    if (shape.txv == 64 && (shape.ty == 16 || shape.ty == 4)) {
-        shape.ty = 8;
-        shape.n_chunks = 0;
-        shape = zq_resolve(d, shape, 0);
-    }
-    return launch_zq(a, d, shape, kAdjoint, st);
+    return launch_zq(a, d, zq_resolve(d, shape, kKindFwiAdjoint), kAdjoint, st);
 }
 
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,

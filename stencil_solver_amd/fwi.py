@@ -21,8 +21,8 @@ import math
 
 import numpy as np
 
-from ._lib import check, lib, stencil_fwi_config
-from .api import GridDims, _stream
+from ._lib import check, lib, stencil_fwi_config, stencil_tile_shape
+from .api import GridDims, _shape_dict, _stream
 
 
 def ricker(n_steps: int, dt: float, f_peak: float, t0: float | None = None, amplitude: float = 1.0) -> np.ndarray:
@@ -39,8 +39,9 @@ class FwiProblem:
     with its device workspace.  ``dtype`` is torch.float32 or torch.float64;
     ``variant`` picks the FP32 step kernels ("tiled": the LDS z-queue kernel,
     "naive": the global-memory one; both give bit-identical results, FP64
-    always uses the latter).  ``tune`` runs the launch-shape autotuner on
-    this grid first (FP32 tiled only); the steps then use the tuned shape."""
+    always uses the latter).  ``tune`` runs the FWI kernels' launch-shape
+    autotuner on this grid first (FP32 tiled only); the steps then use the
+    tuned shapes."""
 
     def __init__(self, grid: GridDims, n_steps: int, src: tuple[int, int, int], rec_z: int, wavelet,
                  device, dtype=None, ckpt_every: int = 0, variant: str = "tiled", tune: bool = False):
@@ -81,19 +82,18 @@ class FwiProblem:
         if tune and self.dtype == torch.float32 and variant == "tiled":
             self.shape = self.tune()
 
-    def tune(self, reps: int = 3) -> dict:
-        """Autotune the tiled step for this grid on scratch fields (a smooth
-        pulse in a constant model); returns the shape the steps will use."""
-        from .api import tune_tiled
-
-        torch = self.torch
-        shape = (self.grid.nz, self.grid.ny, self.grid.nx)
-        f = [torch.zeros(shape, dtype=torch.float32, device=self.device) for _ in range(4)]
-        f[2].fill_(4.0e6)
-        f[0][4:-4, 4:-4, 4:-4] = 1.0
-        best = tune_tiled(f[0], f[1], f[2], f[3], self.grid, reps=reps)
-        del f
-        return best
+    def tune(self, reps: int = 3, csv_path=None) -> dict:
+        """Autotune the FP32 FWI kernels for this grid on the problem's own
+        workspace (a forward pass's checkpoints, traces and stored L p stay
+        intact); returns {"forward": shape, "adjoint": shape}, the shapes the
+        steps then use."""
+        if self.dtype != self.torch.float32 or self.variant != "tiled":
+            raise ValueError("tune() applies to the FP32 tiled FWI kernels")
+        fwd, adj = stencil_tile_shape(), stencil_tile_shape()
+        path = str(csv_path).encode() if csv_path else None
+        check(lib().stencil_tune_fwi_f32(C.byref(self._g), C.byref(self.cfg), self.work.data_ptr(), int(reps),
+                                         C.byref(fwd), C.byref(adj), path, _stream(None)), "tune_fwi")
+        return {"forward": _shape_dict(fwd), "adjoint": _shape_dict(adj)}
 
     @property
     def ckpt_every(self) -> int:

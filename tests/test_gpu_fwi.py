@@ -108,8 +108,10 @@ def test_tiled_fwi_equals_naive(ss, dev, shape, tuned):
 
 
 def test_fwi_problem_tune(ss, dev):
-    """FwiProblem(tune=True) tunes its grid; its traces stay bit-identical to
-    the global-memory kernels' (receiver plane written by the tiled step)."""
+    """FwiProblem(tune=True) tunes both FWI kernels for its grid; its traces
+    and gradient stay bit-identical to the global-memory kernels' (receiver
+    plane written by the tiled step), also when tune() runs between misfit()
+    and gradient()."""
     import torch
 
     shape, n = (36, 40, 104), 24
@@ -119,10 +121,21 @@ def test_fwi_problem_tune(ss, dev):
     md = torch.from_numpy(m).to(dev)
     ss.clear_tuning()
     pt = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, tune=True)
-    assert pt.shape is not None and ss.tuned_shape(grid)[1]
+    assert set(pt.shape) == {"forward", "adjoint"}
+    for kind in ("fwi_forward", "fwi_adjoint"):
+        shp, hit = ss.tuned_shape(grid, kind)
+        assert hit and shp["nontemporal"] == 1
+    assert ss.tuned_shape(grid, "fwi_adjoint")[0] == pt.shape["adjoint"]
     pn = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, variant="naive")
     a, b = pt.traces(md), pn.traces(md)
     assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()) and a.abs().max() > 0
+    # tuning between misfit and gradient leaves the forward pass's state intact
+    d_obs = 0.5 * a
+    pt.misfit(md, d_obs)
+    pt.tune(reps=1)
+    pn.misfit(md, d_obs)
+    ga, gb = pt.gradient(md), pn.gradient(md)
+    assert np.array_equal(ga.cpu().numpy(), gb.cpu().numpy()) and ga.abs().max() > 0
     ss.clear_tuning()
 
 
