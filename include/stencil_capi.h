@@ -145,7 +145,8 @@ int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* gri
                           stencil_tile_shape* best, const char* csv_path, void* hip_stream);
 /* The shape a NULL-shape launch would use for these dims (kind 0 wave, 1
  * heat, 2 + storage for the 16-bit wave step, 6 FWI forward step, 7 adjoint
- * step): returns 1 if it came from the tuner cache, 0 for the default. */
+ * step, 8 forward step recomputed beside the adjoint): returns 1 if it came
+ * from the tuner cache, 0 for the default. */
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
 void stencil_clear_tuning(void);
 
@@ -246,6 +247,10 @@ int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t 
  * it.  With ckpt_every = n_steps (one segment: n_steps fields of workspace,
  * e.g. 134 GB for 256^3 x 2000 FP32) the misfit forward stores every L p^k
  * and the backward runs the adjoint steps only, with the same results.
+ * With more than one segment the backward pass recomputes segment s-1 on a
+ * second stream of its own (created once per device, default priority),
+ * overlapped with the adjoint steps of segment s on hip_stream and joined to
+ * it by events: the call stays asynchronous and its work ends on hip_stream.
  * All device memory is the caller's workspace.  Arrays: vel2 and grad
  * nx*ny*nz, wavelet n_steps (device), traces / d_obs n_steps*ny*nx with trace
  * n holding p^{n+1} on plane rec_z. ---- */
@@ -279,16 +284,20 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
                              const double* wavelet, double* grad, double* src_trace, void* work,
                              void* hip_stream);
 /* Autotuner of the FP32 FWI kernels (no reference counterpart): the sweep of
- * stencil_tune_tiled_f32 applied to the forward step, timed in both of its
- * forms (recording the receiver plane; writing L p), and to the adjoint step,
- * non-temporal streams only.  Runs on the workspace's wavefield and adjoint
- * buffers; the checkpoints, traces and stored L p of an earlier forward pass
- * stay intact.  The fastest shapes are cached for this grid
- * (stencil_tuned_shape kinds 6 and 7) and used by the tiled FWI calls on it.
- * CSV rows: kernel, grid, shape, ms per launch.  Synchronises. */
+ * stencil_tune_tiled_f32 (non-temporal streams only) applied to the forward
+ * step as the forward pass runs it (recording the receiver plane), to the
+ * adjoint step, and to the forward step writing L p while the chosen adjoint
+ * step runs beside it on the second stream, as the backward pass recomputes
+ * (with one segment, or ckpt_every = 1, there is no such overlap and
+ * best_recompute repeats best_forward).  Each timed chain rotates three
+ * buffers like the time loop.  Runs on the workspace's wavefield, adjoint
+ * and Laplacian buffers; the checkpoints, traces and stored L p of an
+ * earlier forward pass stay intact.  The shapes are cached for this grid
+ * (stencil_tuned_shape kinds 6, 7, 8) and used by the tiled FWI calls on
+ * it.  CSV rows: kernel, grid, shape, ms per step.  Synchronises. */
 int stencil_tune_fwi_f32(const stencil_grid* grid, const stencil_fwi_config* cfg, void* work, int reps,
                          stencil_tile_shape* best_forward, stencil_tile_shape* best_adjoint,
-                         const char* csv_path, void* hip_stream);
+                         stencil_tile_shape* best_recompute, const char* csv_path, void* hip_stream);
 /* The single steps: the acoustic step (reference association) plus
  * wavelet[n] at linear cell src_index (< 0: none), writing L p_cur into
  * lap_out (nullable); and the adjoint step in the variable psi = vel2 *

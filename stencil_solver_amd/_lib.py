@@ -117,8 +117,8 @@ SIGNATURES = {
     "stencil_fwi_forward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P, _P]),
     "stencil_fwi_backward_f32": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
     "stencil_fwi_backward_f64": (_I, [_P, _G, C.POINTER(stencil_fwi_config), _P, _P, _P, _P, _P]),
-    "stencil_tune_fwi_f32": (_I, [_G, C.POINTER(stencil_fwi_config), _P, _I, C.POINTER(stencil_tile_shape),
-                                  C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
+    "stencil_tune_fwi_f32": (_I, [_G, C.POINTER(stencil_fwi_config), _P, _I] + [C.POINTER(stencil_tile_shape)] * 3
+                             + [C.c_char_p, _P]),
     "stencil_forward_step_f32": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _I, _P]),
     "stencil_forward_step_f64": (_I, [_P, _P, _P, _P, _P, _G, C.c_int64, _P, C.c_int64, _I, _P]),
     "stencil_adjoint_step_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _G, C.c_int64, _P, _I, _P]),

@@ -12,15 +12,20 @@
 // variable psi = m lambda:
 //   psi^k = 2 psi^{k+1} - psi^{k+2} + dt^2 m L psi^{k+1} + m res^k
 //   grad += dt^2 (L p^k) psi^{k+1},   and at the end grad /= m.
+// The recompute of segment s-1 runs on a second stream, overlapped with the
+// adjoint steps of segment s (at small grids neither kernel fills the GPU).
 // Device memory: all of it is the caller's workspace (stencil_fwi_workspace_bytes):
 // 3 wavefield levels, 3 adjoint levels, 2 * ceil(N/C) checkpoint fields, C
-// Laplacian fields, N receiver planes and the reduction scratch.
+// Laplacian fields (two banks of C with more than one segment), N receiver
+// planes and the reduction scratch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "launchers.hpp"
@@ -63,6 +68,7 @@ struct StepCtx {
     bool tiled = false;  // FP32 on the LDS z-queue kernel
     ZqShape shape{};      // forward steps
     ZqShape adj_shape{};  // adjoint steps
+    ZqShape rec_shape{};  // forward steps recomputed beside the adjoint steps
 };
 
 StepCtx step_ctx(const Dims& d, long long src) {
@@ -79,6 +85,7 @@ StepCtx step_ctx(const Dims& d, long long src) {
     ZqShape tuned;
     if (zq_cached_shape(d, kKindFwiForward, &tuned) || zq_cached_shape(d, 0, &tuned)) c.shape = tuned;
     if (zq_cached_shape(d, kKindFwiAdjoint, &tuned) || zq_cached_shape(d, 0, &tuned)) c.adj_shape = tuned;
+    c.rec_shape = zq_cached_shape(d, kKindFwiRecompute, &tuned) ? tuned : c.shape;
     return c;
 }
 
@@ -87,7 +94,7 @@ bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0;
 struct Plan {
     Dims d;
     size_t cells = 0, plane = 0;
-    int64_t n = 0, C = 0, S = 0;
+    int64_t n = 0, C = 0, S = 0, banks = 1;
     long long src = 0;
     int rec_z = 0;
     // workspace offsets (bytes)
@@ -122,8 +129,11 @@ bool make_plan(const stencil_grid* g, const stencil_fwi_config* c, size_t esz, P
     o += 3 * field;
     p->ckpt = o;
     o += size_t(2 * p->S) * field;
+    // With more than one segment, two banks of C Laplacian fields: the next
+    // segment's recompute fills one while this segment's adjoint reads the other.
+    p->banks = p->S >= 2 ? 2 : 1;
     p->lap = o;
-    o += size_t(p->C) * field;
+    o += size_t(p->banks * p->C) * field;
     p->traces = o;
     o += up(size_t(p->n) * p->plane * esz);
     p->partial = o;
@@ -260,6 +270,46 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
     return STENCIL_OK;
 }
 
+// The recompute stream of the current device: created once (default
+// priority, non-blocking) and kept for the process, so a backward call
+// neither creates nor destroys streams; calls on one device share it.
+hipError_t side_stream(hipStream_t* out) {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = streams.find(dev);
+    if (it == streams.end()) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        it = streams.emplace(dev, st).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+
+// Events ordering the recompute stream against the caller's, destroyed when
+// the backward call returns (the runtime frees them once their work is done).
+struct Events {
+    std::vector<hipEvent_t> ev;
+    ~Events() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t init(size_t n) {
+        ev.reserve(n);
+        for (size_t i = 0; i < n; ++i) {
+            hipEvent_t x = nullptr;
+            const hipError_t e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+            ev.push_back(x);
+        }
+        return hipSuccess;
+    }
+};
+
 template <class T>
 int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, const T* wavelet, T* grad,
              T* src_trace, void* work, hipStream_t st, bool tiled) {
@@ -275,42 +325,84 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     const T* res = at<T>(work, p.traces);
     FWI_TRY(Ops<T>::fill(grad, p.cells, st));
     for (auto* l : lam) FWI_TRY(Ops<T>::fill(l, p.cells, st));
-    int l1 = 0, l2 = 1, lo = 2;  // psi^{k+1}, psi^{k+2}, psi^k (psi = m lambda)
-    const bool lap_stored = p.C == p.n;  // the misfit forward wrote every L p^k
-    // The checkpoint copies restore two levels whole; the steps write only
-    // interiors, so the third level's halo (the zero walls) is cleared here
-    // rather than left to whatever last used the buffer.
-    if (!lap_stored) FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
-    for (int64_t sg = p.S - 1; sg >= 0; --sg) {
-        const int64_t k_lo = sg * p.C + 1;
-        const int64_t k_hi = std::min((sg + 1) * p.C, p.n);
-        // re-run the forward steps of this segment from its checkpoint
-        if (!lap_stored) {
-            const T* ck = at<T>(work, p.ckpt + size_t(2 * sg) * field);
-            FWI_TRY(hipMemcpyAsync(ring[0], ck, p.cells * sizeof(T), hipMemcpyDeviceToDevice, st));
-            FWI_TRY(hipMemcpyAsync(ring[1], reinterpret_cast<const char*>(ck) + field, p.cells * sizeof(T),
-                                   hipMemcpyDeviceToDevice, st));
-        }
-        // L p^N would only meet lambda^{N+1} = 0, so the last segment stops at
-        // step N-1 (wavelet has N samples) and its k = N adjoint step skips
-        // the gradient term.
+    // L p^k of segment sg, k = sg*C + 1 + i, in bank sg % banks
+    auto lap_slot = [&](int64_t sg, int64_t i) {
+        return at<T>(work, p.lap + size_t((sg % p.banks) * p.C + i) * field);
+    };
+    auto k_lo = [&](int64_t sg) { return sg * p.C + 1; };
+    auto k_hi = [&](int64_t sg) { return std::min((sg + 1) * p.C, p.n); };
+    // Re-run the forward steps of segment sg from its checkpoint, writing
+    // L p^k; L p^N would only meet lambda^{N+1} = 0, so the last segment
+    // stops at step N-1 (the wavelet has N samples).
+    StepCtx rcx = cx;
+    rcx.shape = cx.rec_shape;
+    auto recompute = [&](int64_t sg, hipStream_t rs) -> hipError_t {
+        const T* ck = at<T>(work, p.ckpt + size_t(2 * sg) * field);
+        hipError_t e = hipMemcpyAsync(ring[0], ck, p.cells * sizeof(T), hipMemcpyDeviceToDevice, rs);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ring[1], reinterpret_cast<const char*>(ck) + field, p.cells * sizeof(T),
+                               hipMemcpyDeviceToDevice, rs);
         int prev = 0, cur = 1, next = 2;
-        for (int64_t j = k_lo; !lap_stored && j <= std::min(k_hi, p.n - 1); ++j) {
-            T* lap = at<T>(work, p.lap + size_t(j - k_lo) * field);
-            FWI_TRY(Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap, cx, s, wavelet, int(j), nullptr, 0, st));
+        for (int64_t j = k_lo(sg); e == hipSuccess && j <= std::min(k_hi(sg), p.n - 1); ++j) {
+            e = Ops<T>::fwd(ring[cur], ring[prev], m, ring[next], lap_slot(sg, j - k_lo(sg)), rcx, s, wavelet,
+                            int(j), nullptr, 0, rs);
             const int t = prev;
             prev = cur;
             cur = next;
             next = t;
         }
-        for (int64_t k = k_hi; k >= k_lo; --k) {
-            const T* lap = k < p.n ? at<T>(work, p.lap + size_t(k - k_lo) * field) : nullptr;
-            FWI_TRY(Ops<T>::adj(lam[l1], lam[l2], m, lap, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z,
-                                cx, s, src_trace ? src_trace + (k - 1) : nullptr, st));
+        return e;
+    };
+    // The adjoint steps k = k_hi .. k_lo of segment sg (the k = N step skips
+    // the gradient term).
+    int l1 = 0, l2 = 1, lo = 2;  // psi^{k+1}, psi^{k+2}, psi^k (psi = m lambda)
+    auto adjoint = [&](int64_t sg) -> hipError_t {
+        hipError_t e = hipSuccess;
+        for (int64_t k = k_hi(sg); e == hipSuccess && k >= k_lo(sg); --k) {
+            const T* lap = k < p.n ? lap_slot(sg, k - k_lo(sg)) : nullptr;
+            e = Ops<T>::adj(lam[l1], lam[l2], m, lap, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z, cx, s,
+                            src_trace ? src_trace + (k - 1) : nullptr, st);
             const int t = l2;
             l2 = l1;
             l1 = lo;
             lo = t;
+        }
+        return e;
+    };
+    if (p.S == 1) {
+        // one segment: the misfit forward wrote every L p^k (bank 0)
+        FWI_TRY(adjoint(0));
+    } else {
+        // Segments last to first; the recompute of segment sg-1 runs on a
+        // second stream while the adjoint steps of segment sg run on the
+        // caller's (the two read and write disjoint buffers; at small grids
+        // neither kernel fills the GPU alone).  Events: start, then
+        // "recomputed" and "adjoint done" per bank, re-recorded segment
+        // after segment (a wait applies to the record made before it).
+        hipStream_t rs = nullptr;
+        FWI_TRY(side_stream(&rs));
+        Events evs;
+        FWI_TRY(evs.init(5));
+        hipEvent_t start = evs.ev[0];
+        auto recomputed = [&](int64_t sg) { return evs.ev[size_t(1 + sg % 2)]; };
+        auto adjoint_done = [&](int64_t sg) { return evs.ev[size_t(3 + sg % 2)]; };
+        // The steps write interiors only: the third level's halo (the zero
+        // walls) is cleared here rather than left to the buffer's last user.
+        FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
+        FWI_TRY(hipEventRecord(start, st));
+        FWI_TRY(hipStreamWaitEvent(rs, start, 0));
+        FWI_TRY(recompute(p.S - 1, rs));
+        FWI_TRY(hipEventRecord(recomputed(p.S - 1), rs));
+        for (int64_t sg = p.S - 1; sg >= 0; --sg) {
+            if (sg >= 1) {
+                // bank (sg-1) % 2 was last read by the adjoint steps of sg+1
+                if (sg + 1 <= p.S - 1) FWI_TRY(hipStreamWaitEvent(rs, adjoint_done(sg + 1), 0));
+                FWI_TRY(recompute(sg - 1, rs));
+                FWI_TRY(hipEventRecord(recomputed(sg - 1), rs));
+            }
+            FWI_TRY(hipStreamWaitEvent(st, recomputed(sg), 0));
+            FWI_TRY(adjoint(sg));
+            FWI_TRY(hipEventRecord(adjoint_done(sg), st));
         }
     }
     // grad held sum_k dt^2 (L p^k) psi^{k+1}; lambda = psi / m
@@ -327,57 +419,78 @@ void put_shape(const ZqShape& z, stencil_tile_shape* out) {
     out->nontemporal = z.nt;
 }
 
-// Launch-shape sweep of the two FP32 FWI kernels on the workspace's wavefield
-// and adjoint buffers: the forward step in both of its forms (recording the
-// receiver plane, as the forward pass runs it; writing L p, as the segment
-// recompute does) and the adjoint step, over the tuner's shapes with
-// non-temporal streams, the only mode these kernels run in.
+// Launch-shape sweep of the FP32 FWI kernels on the workspace's wavefield,
+// adjoint and (with more than one segment) Laplacian buffers, over the
+// tuner's shapes with non-temporal streams (the only mode these kernels run
+// in).  Each timed chain rotates three buffers as the leapfrog loops do, so
+// the sweep-direction alternation between consecutive steps is part of what
+// is timed: the forward step recording the receiver plane (the forward pass),
+// the adjoint step alone, and the forward step writing L p while the best
+// adjoint step runs beside it on the recompute stream (the backward pass).
 int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int reps, stencil_tile_shape* best_fwd,
-         stencil_tile_shape* best_adj, const char* csv_path, hipStream_t st) {
+         stencil_tile_shape* best_adj, stencil_tile_shape* best_rec, const char* csv_path, hipStream_t st) {
     Plan p;
     if (!work || reps <= 0 || !make_plan(g, c, sizeof(float), &p)) return STENCIL_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(work) % kAlign != 0) return STENCIL_ERR_INVALID;
     const size_t field = up(p.cells * sizeof(float));
-    float* m = at<float>(work, p.ring);
-    float* f1 = at<float>(work, p.ring + field);
-    float* f2 = at<float>(work, p.ring + 2 * field);
-    float* o0 = at<float>(work, p.lam);
-    float* o1 = at<float>(work, p.lam + field);
-    float* o2 = at<float>(work, p.lam + 2 * field);
+    // Scratch fields: 3 wavefield and 3 adjoint levels, then the Laplacian
+    // slots, which hold nothing between calls when there are two banks.
+    std::vector<float*> f;
+    for (int i = 0; i < 3; ++i) f.push_back(at<float>(work, p.ring + size_t(i) * field));
+    for (int i = 0; i < 3; ++i) f.push_back(at<float>(work, p.lam + size_t(i) * field));
+    if (p.banks == 2)
+        for (int64_t i = 0; i < std::min<int64_t>(p.banks * p.C, 4); ++i)
+            f.push_back(at<float>(work, p.lap + size_t(i) * field));
+    const bool pair = f.size() >= 10;
     float* trace = at<float>(work, p.partial);  // reduction scratch
-    const void* ptrs[4] = {m, f1, f2, o0};
+    const void* ptrs[4] = {f[0], f[1], f[2], f[3]};
     if (!zq_supported(p.d, ptrs, 4)) return STENCIL_ERR_UNSUPPORTED;
-    // (2000 m/s)^2 and fields of ones
+    float* m = f[0];  // (2000 m/s)^2; every other field ones
     FWI_TRY(fill_f32(m, p.cells, 4.0e6f, st));
-    for (float* f : {f1, f2, o0, o1, o2}) FWI_TRY(fill_f32(f, p.cells, 1.0f, st));
+    for (size_t i = 1; i < f.size(); ++i) FWI_TRY(fill_f32(f[i], p.cells, 1.0f, st));
     const Scalars<float> s = scalars<float>(g);
     const StepCtx cx = step_ctx(p.d, p.src);
+    hipStream_t rs = nullptr;
+    Events evs;
+    if (pair) {
+        FWI_TRY(side_stream(&rs));
+        FWI_TRY(evs.init(2));
+    }
+    // step i of a chain over b[0..2]: prev b[i % 3], cur b[(i+1) % 3], next b[(i+2) % 3]
+    struct Chain {
+        float* b[3];
+        int i = 0;
+        float* prev() const { return b[i % 3]; }
+        float* cur() const { return b[(i + 1) % 3]; }
+        float* next() const { return b[(i + 2) % 3]; }
+    };
+    auto fwd = [&](Chain& ch, const ZqShape& sh, float* lap, float* rec, hipStream_t q) {
+        const hipError_t e = wave_zq_fwi_f32(ch.cur(), ch.prev(), m, ch.next(), lap, cx.src_z, cx.src_y, cx.src_x, m,
+                                             rec, p.rec_z, p.d, s, sh, q);
+        ++ch.i;
+        return e;
+    };
+    auto adj = [&](Chain& ch, const ZqShape& sh, const float* lap, float* grad, const float* res) {
+        const hipError_t e = adjoint_zq_f32(ch.cur(), ch.prev(), m, lap, grad, ch.next(), res, p.rec_z, cx.src_z,
+                                            cx.src_y, cx.src_x, trace, p.d, s, sh, st);
+        ++ch.i;
+        return e;
+    };
     struct Row {
         ZqShape shape;
-        float fwd_ms, adj_ms;
+        float fwd_ms, adj_ms, rec_ms;
     };
     std::vector<Row> rows;
     for (const ZqShape& sh : zq_tune_candidates(p.d)) {
         if (sh.nt != 1) continue;
-        Row r{sh, -1.f, -1.f};
-        const float pair = zq_time_launches(
-            [&] {
-                const hipError_t e = wave_zq_fwi_f32(f1, f2, m, o0, nullptr, cx.src_z, cx.src_y, cx.src_x, f1, o2,
-                                                     p.rec_z, p.d, s, sh, st);
-                if (e != hipSuccess) return e;
-                return wave_zq_fwi_f32(f1, f2, m, o0, o1, cx.src_z, cx.src_y, cx.src_x, f1, nullptr, 0, p.d, s, sh,
-                                       st);
-            },
-            reps, st);
-        if (pair > 0.f) r.fwd_ms = 0.5f * pair;
+        Row r{sh, -1.f, -1.f, -1.f};
+        Chain a{{f[1], f[2], f[3]}};
+        r.fwd_ms = zq_time_launches([&] { return fwd(a, sh, nullptr, f[5], st); }, reps, st);
         // the adjoint runs 64 x 4 and 64 x 16 as 64 x 8: not timed twice
-        if (!(sh.txv == 64 && (sh.ty == 4 || sh.ty == 16)))
-            r.adj_ms = zq_time_launches(
-                [&] {
-                    return adjoint_zq_f32(f1, f2, m, o1, o2, o0, f1, p.rec_z, cx.src_z, cx.src_y, cx.src_x, trace,
-                                          p.d, s, sh, st);
-                },
-                reps, st);
+        if (!(sh.txv == 64 && (sh.ty == 4 || sh.ty == 16))) {
+            Chain b{{f[1], f[2], f[3]}};
+            r.adj_ms = zq_time_launches([&] { return adj(b, sh, f[4], f[5], f[4]); }, reps, st);
+        }
         rows.push_back(r);
     }
     const Row* bf = nullptr;
@@ -387,23 +500,52 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
         if (r.adj_ms > 0.f && (!ba || r.adj_ms < ba->adj_ms)) ba = &r;
     }
     if (!bf || !ba) return STENCIL_ERR_HIP;
-    zq_cache_put(p.d, kKindFwiForward, bf->shape);
-    zq_cache_put(p.d, kKindFwiAdjoint, ba->shape);
-    put_shape(bf->shape, best_fwd);
-    put_shape(ba->shape, best_adj);
-    if (csv_path && *csv_path) {
-        FILE* f = std::fopen(csv_path, "w");
-        if (!f) return STENCIL_ERR_INVALID;
-        std::fprintf(f, "kernel,nx,ny,nz,tile_x,tile_y,n_chunks,xcd_remap,ms\n");
+    const ZqShape adj_best = ba->shape;
+    const Row* br = nullptr;
+    if (pair) {
+        // recompute chain f[1..3] (L p into f[4]) on the recompute stream, the
+        // adjoint chain f[6..8] (L p from f[5], gradient f[9]) on st, joined
+        // every kSteps steps (the backward pass joins once per segment)
+        constexpr int kSteps = 8;
+        for (Row& r : rows) {
+            Chain a{{f[1], f[2], f[3]}};
+            Chain b{{f[6], f[7], f[8]}};
+            const float ms = zq_time_launches(
+                [&] {
+                    hipError_t e = hipEventRecord(evs.ev[0], st);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(rs, evs.ev[0], 0);
+                    for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = fwd(a, r.shape, f[4], nullptr, rs);
+                    if (e == hipSuccess) e = hipEventRecord(evs.ev[1], rs);
+                    for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = adj(b, adj_best, f[5], f[9], f[5]);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(st, evs.ev[1], 0);
+                    return e;
+                },
+                reps, st);
+            if (ms > 0.f) r.rec_ms = ms / kSteps;
+        }
         for (const Row& r : rows)
-            for (int k = 0; k < 2; ++k) {
-                const float ms = k == 0 ? r.fwd_ms : r.adj_ms;
+            if (r.rec_ms > 0.f && (!br || r.rec_ms < br->rec_ms)) br = &r;
+    }
+    zq_cache_put(p.d, kKindFwiForward, bf->shape);
+    zq_cache_put(p.d, kKindFwiAdjoint, adj_best);
+    put_shape(bf->shape, best_fwd);
+    put_shape(adj_best, best_adj);
+    if (br) zq_cache_put(p.d, kKindFwiRecompute, br->shape);
+    put_shape(br ? br->shape : bf->shape, best_rec);
+    if (csv_path && *csv_path) {
+        FILE* out = std::fopen(csv_path, "w");
+        if (!out) return STENCIL_ERR_INVALID;
+        std::fprintf(out, "kernel,nx,ny,nz,tile_x,tile_y,n_chunks,xcd_remap,ms\n");
+        static const char* const names[3] = {"forward", "adjoint", "recompute_beside_adjoint"};
+        for (const Row& r : rows)
+            for (int k = 0; k < 3; ++k) {
+                const float ms = k == 0 ? r.fwd_ms : k == 1 ? r.adj_ms : r.rec_ms;
                 if (ms > 0.f)
-                    std::fprintf(f, "%s,%lld,%lld,%lld,%d,%d,%d,%d,%.5f\n", k == 0 ? "forward" : "adjoint",
-                                 (long long)p.d.nx, (long long)p.d.ny, (long long)p.d.nz, 4 * r.shape.txv,
-                                 r.shape.ty, r.shape.n_chunks, r.shape.xcd_remap, ms);
+                    std::fprintf(out, "%s,%lld,%lld,%lld,%d,%d,%d,%d,%.5f\n", names[k], (long long)p.d.nx,
+                                 (long long)p.d.ny, (long long)p.d.nz, 4 * r.shape.txv, r.shape.ty, r.shape.n_chunks,
+                                 r.shape.xcd_remap, ms);
             }
-        std::fclose(f);
+        std::fclose(out);
     }
     return STENCIL_OK;
 }
@@ -442,9 +584,10 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
                             false);
 }
 int stencil_tune_fwi_f32(const stencil_grid* grid, const stencil_fwi_config* cfg, void* work, int reps,
-                         stencil_tile_shape* best_forward, stencil_tile_shape* best_adjoint, const char* csv_path,
-                         void* stream) {
-    return tune(grid, cfg, work, reps, best_forward, best_adjoint, csv_path, static_cast<hipStream_t>(stream));
+                         stencil_tile_shape* best_forward, stencil_tile_shape* best_adjoint,
+                         stencil_tile_shape* best_recompute, const char* csv_path, void* stream) {
+    return tune(grid, cfg, work, reps, best_forward, best_adjoint, best_recompute, csv_path,
+                static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -51,6 +51,7 @@ bool zq_shape_valid(int txv, int ty);
 // step, then the two FWI kernels.
 constexpr int kKindFwiForward = 6;
 constexpr int kKindFwiAdjoint = 7;
+constexpr int kKindFwiRecompute = 8;  // forward steps beside the adjoint steps
 // Fills unset fields of a shape for a kernel of this kind; the adjoint kind
 // also maps the tiles it is not built for (64 x 4, 64 x 16) to 64 x 8.
 ZqShape zq_resolve(const Dims& d, ZqShape s, int kind);

@@ -1,4 +1,4 @@
-// 16-bit field conversion kernels and the FP64 reductions used to judge the
+// 16-bit field conversion kernels and the FP64 reductions used to measure the errors of the
 // reduced-precision and adjoint paths (energy norm, relative L2 against an
 // FP64 run, dot products).
 //

@@ -222,7 +222,7 @@ int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stenc
 }
 
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out) {
-    if (!dims_ok(grid) || !out || kind < 0 || kind > kKindFwiAdjoint) return STENCIL_ERR_INVALID;
+    if (!dims_ok(grid) || !out || kind < 0 || kind > kKindFwiRecompute) return STENCIL_ERR_INVALID;
     Dims d;
     d.nx = int64_t(grid->nx);
     d.ny = int64_t(grid->ny);

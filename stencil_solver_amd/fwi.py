@@ -85,15 +85,16 @@ class FwiProblem:
     def tune(self, reps: int = 3, csv_path=None) -> dict:
         """Autotune the FP32 FWI kernels for this grid on the problem's own
         workspace (a forward pass's checkpoints, traces and stored L p stay
-        intact); returns {"forward": shape, "adjoint": shape}, the shapes the
-        steps then use."""
+        intact); returns {"forward", "adjoint", "recompute"} shapes: the
+        forward pass's steps, the adjoint steps, and the forward steps the
+        backward pass recomputes beside them."""
         if self.dtype != self.torch.float32 or self.variant != "tiled":
             raise ValueError("tune() applies to the FP32 tiled FWI kernels")
-        fwd, adj = stencil_tile_shape(), stencil_tile_shape()
+        shp = [stencil_tile_shape() for _ in range(3)]
         path = str(csv_path).encode() if csv_path else None
         check(lib().stencil_tune_fwi_f32(C.byref(self._g), C.byref(self.cfg), self.work.data_ptr(), int(reps),
-                                         C.byref(fwd), C.byref(adj), path, _stream(None)), "tune_fwi")
-        return {"forward": _shape_dict(fwd), "adjoint": _shape_dict(adj)}
+                                         *[C.byref(x) for x in shp], path, _stream(None)), "tune_fwi")
+        return dict(zip(("forward", "adjoint", "recompute"), (_shape_dict(x) for x in shp)))
 
     @property
     def ckpt_every(self) -> int:

@@ -121,11 +121,11 @@ def test_fwi_problem_tune(ss, dev):
     md = torch.from_numpy(m).to(dev)
     ss.clear_tuning()
     pt = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, tune=True)
-    assert set(pt.shape) == {"forward", "adjoint"}
-    for kind in ("fwi_forward", "fwi_adjoint"):
+    assert set(pt.shape) == {"forward", "adjoint", "recompute"}
+    for kind in ("fwi_forward", "fwi_adjoint", "fwi_recompute"):
         shp, hit = ss.tuned_shape(grid, kind)
         assert hit and shp["nontemporal"] == 1
-    assert ss.tuned_shape(grid, "fwi_adjoint")[0] == pt.shape["adjoint"]
+        assert shp == pt.shape[kind[4:]]
     pn = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, variant="naive")
     a, b = pt.traces(md), pn.traces(md)
     assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()) and a.abs().max() > 0
