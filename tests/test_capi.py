@@ -109,3 +109,47 @@ def test_host_generators_equal_oracle(oracle):
     assert np.array_equal(ss.host_layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.05, 2),
                           oracle.layered_noise_vel2(shape, 16, 1500.0, 4000.0, 0.05, 2))
     assert np.array_equal(ss.host_noise_cube(shape, 8, 10.0, 5), oracle.heat_noise_cube(shape, 8, 10.0, 5))
+
+
+@pytest.mark.parametrize("n,ckpt,esz", [(24, 0, 4), (24, 24, 4), (2000, 0, 4), (37, 1, 8), (10, 7, 8)])
+def test_fwi_workspace_layout(n, ckpt, esz):
+    """Workspace bytes: 3 wavefield and 3 adjoint levels, 2 * S checkpoint
+    fields (S = ceil(n / C) segments), two banks of C Laplacian fields when
+    S > 1 (the next segment's recompute overlaps this one's adjoint steps),
+    one bank when S = 1, then n receiver planes and the reduction scratch,
+    each region 256-byte aligned."""
+    import math
+
+    import stencil_solver_amd as ss
+    from stencil_solver_amd._lib import stencil_fwi_config, stencil_grid
+
+    nx, ny, nz = 44, 36, 30
+    g = stencil_grid(nx, ny, nz, 10.0, 10.0, 10.0, 1e-3)
+    cfg = stencil_fwi_config(n, 20, 18, 15, 6, ckpt, 1)
+    up = lambda b: (b + 255) // 256 * 256
+    field = up(nx * ny * nz * esz)
+    C_ = min(ckpt or math.ceil(math.sqrt(n)), n)
+    S = (n + C_ - 1) // C_
+    banks = 2 if S > 1 else 1
+    want = 6 * field + 2 * S * field + banks * C_ * field + up(n * nx * ny * esz) + up(1024 * 8) + up(8)
+    assert ss.lib().stencil_fwi_workspace_bytes(C.byref(g), C.byref(cfg), esz) == want
+    # invalid: receiver plane in the halo, value size 2
+    bad = stencil_fwi_config(n, 20, 18, 15, 2, ckpt, 1)
+    assert ss.lib().stencil_fwi_workspace_bytes(C.byref(g), C.byref(bad), esz) == 0
+    assert ss.lib().stencil_fwi_workspace_bytes(C.byref(g), C.byref(cfg), 2) == 0
+
+
+def test_fwi_tuner_rejects_invalid_arguments_without_device():
+    import stencil_solver_amd as ss
+    from stencil_solver_amd._lib import stencil_fwi_config, stencil_grid
+
+    L = ss.lib()
+    g = stencil_grid(44, 36, 30, 10.0, 10.0, 10.0, 1e-3)
+    cfg = stencil_fwi_config(24, 20, 18, 15, 6, 0, 1)
+    work = C.c_void_p(0x10000)
+    assert L.stencil_tune_fwi_f32(C.byref(g), C.byref(cfg), None, 3, None, None, None, None, None) == 1
+    assert L.stencil_tune_fwi_f32(C.byref(g), C.byref(cfg), work, 0, None, None, None, None, None) == 1
+    assert L.stencil_tune_fwi_f32(C.byref(g), C.byref(cfg), C.c_void_p(0x10010), 3, None, None, None, None,
+                                  None) == 1  # work not 256-byte aligned
+    bad = stencil_fwi_config(24, 2, 18, 15, 6, 0, 1)  # source in the halo
+    assert L.stencil_tune_fwi_f32(C.byref(g), C.byref(bad), work, 3, None, None, None, None, None) == 1
