@@ -376,16 +376,15 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
         // Segments last to first; the recompute of segment sg-1 runs on a
         // second stream while the adjoint steps of segment sg run on the
         // caller's (the two read and write disjoint buffers; at small grids
-        // neither kernel fills the GPU alone).  Events: start, then
-        // "recomputed" and "adjoint done" per bank, re-recorded segment
-        // after segment (a wait applies to the record made before it).
+        // neither kernel fills the GPU alone).  Events, each recorded once:
+        // start, then "recomputed" and "adjoint done" per segment.
         hipStream_t rs = nullptr;
         FWI_TRY(side_stream(&rs));
         Events evs;
-        FWI_TRY(evs.init(5));
+        FWI_TRY(evs.init(size_t(1 + 2 * p.S)));
         hipEvent_t start = evs.ev[0];
-        auto recomputed = [&](int64_t sg) { return evs.ev[size_t(1 + sg % 2)]; };
-        auto adjoint_done = [&](int64_t sg) { return evs.ev[size_t(3 + sg % 2)]; };
+        auto recomputed = [&](int64_t sg) { return evs.ev[size_t(1 + sg)]; };
+        auto adjoint_done = [&](int64_t sg) { return evs.ev[size_t(1 + p.S + sg)]; };
         // The steps write interiors only: the third level's halo (the zero
         // walls) is cleared here rather than left to the buffer's last user.
         FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));
@@ -451,11 +450,7 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     const Scalars<float> s = scalars<float>(g);
     const StepCtx cx = step_ctx(p.d, p.src);
     hipStream_t rs = nullptr;
-    Events evs;
-    if (pair) {
-        FWI_TRY(side_stream(&rs));
-        FWI_TRY(evs.init(2));
-    }
+    if (pair) FWI_TRY(side_stream(&rs));
     // step i of a chain over b[0..2]: prev b[i % 3], cur b[(i+1) % 3], next b[(i+2) % 3]
     struct Chain {
         float* b[3];
@@ -506,18 +501,25 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
         // recompute chain f[1..3] (L p into f[4]) on the recompute stream, the
         // adjoint chain f[6..8] (L p from f[5], gradient f[9]) on st, joined
         // every kSteps steps (the backward pass joins once per segment)
+        // (a fork and a join event per timed call, each recorded once)
         constexpr int kSteps = 8;
         for (Row& r : rows) {
             Chain a{{f[1], f[2], f[3]}};
             Chain b{{f[6], f[7], f[8]}};
+            Events evs;
+            FWI_TRY(evs.init(size_t(2 * (reps + 2))));
+            size_t call = 0;
             const float ms = zq_time_launches(
                 [&] {
-                    hipError_t e = hipEventRecord(evs.ev[0], st);
-                    if (e == hipSuccess) e = hipStreamWaitEvent(rs, evs.ev[0], 0);
+                    if (2 * call + 1 >= evs.ev.size()) return hipErrorInvalidValue;  // more calls than events
+                    hipEvent_t fork = evs.ev[2 * call], join = evs.ev[2 * call + 1];
+                    ++call;
+                    hipError_t e = hipEventRecord(fork, st);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(rs, fork, 0);
                     for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = fwd(a, r.shape, f[4], nullptr, rs);
-                    if (e == hipSuccess) e = hipEventRecord(evs.ev[1], rs);
+                    if (e == hipSuccess) e = hipEventRecord(join, rs);
                     for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = adj(b, adj_best, f[5], f[9], f[5]);
-                    if (e == hipSuccess) e = hipStreamWaitEvent(st, evs.ev[1], 0);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
                     return e;
                 },
                 reps, st);
