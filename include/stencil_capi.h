@@ -148,6 +148,11 @@ int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* gri
  * step, 8 forward step recomputed beside the adjoint): returns 1 if it came
  * from the tuner cache, 0 for the default. */
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
+/* Puts a shape into the tuner cache for these dims and kind (as above; kind 2
+ * is unused), e.g. one saved from an earlier run's tuning; zero n_chunks and
+ * -1 flags stay automatic.  STENCIL_ERR_INVALID for a shape the library is
+ * not built with. */
+int stencil_set_tuned_shape(const stencil_grid* grid, int kind, const stencil_tile_shape* shape);
 void stencil_clear_tuning(void);
 
 /* HBM calibration (the reference's tests/perf/bench_stream.cu:53-101):

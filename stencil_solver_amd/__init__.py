@@ -33,6 +33,7 @@ from .api import (  # noqa: F401
     tune_heat,
     tune_tiled,
     tuned_shape,
+    set_tuned_shape,
     clear_tuning,
     upload_fd_coefficients,
     wave_bytes_per_step,

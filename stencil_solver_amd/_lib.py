@@ -92,6 +92,7 @@ SIGNATURES = {
     "stencil_tune_tiled_f32": (_I, [_P, _P, _P, _P, _G, _I, C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
     "stencil_tune_heat_f32": (_I, [_P, _P, _G, _I, C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
     "stencil_tuned_shape": (_I, [_G, _I, C.POINTER(stencil_tile_shape)]),
+    "stencil_set_tuned_shape": (_I, [_G, _I, C.POINTER(stencil_tile_shape)]),
     "stencil_clear_tuning": (None, []),
     "stencil_field16_bytes": (_I, [_I, _G, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "stencil_encode16": (_I, [_I, _P, C.POINTER(stencil_field16), C.c_uint64, C.c_float, _P]),

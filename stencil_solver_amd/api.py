@@ -200,18 +200,33 @@ def tune_heat(t, t_next, grid: GridDims, reps: int = 5, csv_path=None, stream=No
     return _shape_dict(shp)
 
 
+# Tuner-cache kinds (the C-ABI's kind numbers).
+_SHAPE_KINDS = {"wave": 0, "heat": 1, "bf16": 3, "fp16": 4, "bfp16": 5, "fwi_forward": 6, "fwi_adjoint": 7,
+                "fwi_recompute": 8}
+
+
 def tuned_shape(grid: GridDims, kind: str = "wave"):
     """(shape dict, from_tuner) a NULL-shape launch would use on these dims;
     kind "wave", "heat", a 16-bit storage name for launch_tiled_16, or
     "fwi_forward" / "fwi_adjoint" / "fwi_recompute" for the FWI steps."""
     shp = stencil_tile_shape()
     g = grid.to_c()
-    kinds = {"wave": 0, "heat": 1, "bf16": 3, "fp16": 4, "bfp16": 5, "fwi_forward": 6, "fwi_adjoint": 7,
-             "fwi_recompute": 8}
-    r = lib().stencil_tuned_shape(C.byref(g), kinds[kind], C.byref(shp))
+    r = lib().stencil_tuned_shape(C.byref(g), _SHAPE_KINDS[kind], C.byref(shp))
     if r < 0 or r > 1:
         check(r, "tuned_shape")
     return _shape_dict(shp), bool(r)
+
+
+def set_tuned_shape(grid: GridDims, kind: str, shape: dict) -> None:
+    """Put a launch shape (a dict as tuned_shape returns; tile_x in cells)
+    into the tuner cache for these dims and kind, e.g. one saved from an
+    earlier run's tuning."""
+    if shape["tile_x"] % 4:
+        raise ValueError("tile_x must be a multiple of 4 cells")
+    shp = stencil_tile_shape(shape["tile_x"] // 4, shape["tile_y"], shape.get("n_chunks", 0),
+                             shape.get("xcd_remap", -1), shape.get("nontemporal", -1))
+    g = grid.to_c()
+    check(lib().stencil_set_tuned_shape(C.byref(g), _SHAPE_KINDS[kind], C.byref(shp)), "set_tuned_shape")
 
 
 def clear_tuning() -> None:

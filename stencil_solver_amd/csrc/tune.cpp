@@ -238,6 +238,25 @@ int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* 
     return hit ? 1 : 0;
 }
 
+int stencil_set_tuned_shape(const stencil_grid* grid, int kind, const stencil_tile_shape* shape) {
+    if (!dims_ok(grid) || !shape || kind < 0 || kind > kKindFwiRecompute || kind == 2) return STENCIL_ERR_INVALID;
+    if (!zq_shape_valid(shape->tile_x4, shape->tile_y) || shape->n_chunks < 0 || shape->xcd_remap < -1 ||
+        shape->xcd_remap > 1 || shape->nontemporal < -1 || shape->nontemporal > 1)
+        return STENCIL_ERR_INVALID;
+    Dims d;
+    d.nx = int64_t(grid->nx);
+    d.ny = int64_t(grid->ny);
+    d.nz = int64_t(grid->nz);
+    ZqShape s;
+    s.txv = shape->tile_x4;
+    s.ty = shape->tile_y;
+    s.n_chunks = shape->n_chunks;
+    s.xcd_remap = shape->xcd_remap;
+    s.nt = shape->nontemporal;
+    zq_cache_put(d, kind, s);
+    return STENCIL_OK;
+}
+
 void stencil_clear_tuning(void) { zq_cache_clear(); }
 
 }  // extern "C"

@@ -153,3 +153,23 @@ def test_fwi_tuner_rejects_invalid_arguments_without_device():
                                   None) == 1  # work not 256-byte aligned
     bad = stencil_fwi_config(24, 2, 18, 15, 6, 0, 1)  # source in the halo
     assert L.stencil_tune_fwi_f32(C.byref(g), C.byref(bad), work, 3, None, None, None, None, None) == 1
+
+
+def test_set_tuned_shape_round_trip_and_validation():
+    import stencil_solver_amd as ss
+
+    g = ss.GridDims(72, 44, 30)
+    ss.clear_tuning()
+    shp = {"tile_x": 128, "tile_y": 16, "n_chunks": 3, "xcd_remap": 0, "nontemporal": 1}
+    for kind in ("wave", "heat", "bfp16", "fwi_forward", "fwi_adjoint", "fwi_recompute"):
+        ss.set_tuned_shape(g, kind, shp)
+        got, hit = ss.tuned_shape(g, kind)
+        assert hit and got == shp, kind
+    from stencil_solver_amd._lib import StencilError
+
+    with pytest.raises(StencilError):
+        ss.set_tuned_shape(g, "wave", dict(shp, tile_y=7))  # not a built tile
+    with pytest.raises(StencilError):
+        ss.set_tuned_shape(g, "wave", dict(shp, xcd_remap=2))
+    ss.clear_tuning()
+    assert not ss.tuned_shape(g, "fwi_adjoint")[1]

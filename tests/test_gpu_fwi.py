@@ -107,6 +107,47 @@ def test_tiled_fwi_equals_naive(ss, dev, shape, tuned):
     ss.clear_tuning()
 
 
+@pytest.mark.parametrize("shape", [(30, 44, 72), (41, 20, 36)])
+def test_fwi_every_tile_shape_equals_naive(ss, dev, shape):
+    """The FP32 FWI forward, recompute and adjoint kernels at every built tile
+    shape, 1 to 3 z-chunks and both workgroup orders (the three kernels given
+    different shapes), against the global-memory kernels on ragged grids:
+    traces, misfit, gradient and source trace bit-identical.  Four segments,
+    so the backward pass overlaps recomputes with adjoint steps."""
+    import torch
+
+    n = 24
+    src, rec_z = (shape[0] // 2, shape[1] // 2, shape[2] // 2), 5
+    m = smooth_model(shape, 11, 1800.0, 3000.0, np.float32)
+    w = (1e3 * ss.ricker(n, 1e-3, 20.0)).astype(np.float32)
+    grid = _grid(ss, shape, 10.0, 1e-3, np.float32)
+    md = torch.from_numpy(m).to(dev)
+    ss.clear_tuning()
+    pn = ss.FwiProblem(grid, n, src, rec_z, w, dev, torch.float32, 6, "naive")
+    d_obs = pn.traces(md * 1.05)
+    J_n = pn.misfit(md, d_obs).item()
+    g_n, st_n = (t.cpu().numpy() for t in pn.gradient(md, src_trace=True))
+    assert np.abs(g_n).max() > 0
+    pt = ss.FwiProblem(grid, n, src, rec_z, w, dev, torch.float32, 6, "tiled")
+    tiles = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]  # float4 x rows
+    combos = [(t, c, r) for t in tiles for c in (1, 2, 3) for r in (0, 1)]
+    for i, (t, c, r) in enumerate(combos):
+        def shp(j):
+            (tx, ty), cj, rj = combos[j % len(combos)]
+            return {"tile_x": 4 * tx, "tile_y": ty, "n_chunks": cj, "xcd_remap": rj, "nontemporal": 1}
+        ss.clear_tuning()
+        ss.set_tuned_shape(grid, "fwi_forward", shp(i))
+        ss.set_tuned_shape(grid, "fwi_adjoint", shp(i + 17))
+        ss.set_tuned_shape(grid, "fwi_recompute", shp(i + 31))
+        J = pt.misfit(md, d_obs).item()
+        g, st = pt.gradient(md, src_trace=True)
+        assert J == J_n, (i, shp(i))
+        assert np.array_equal(g.cpu().numpy(), g_n), (i, shp(i), shp(i + 17), shp(i + 31))
+        assert np.array_equal(st.cpu().numpy(), st_n), i
+    assert np.array_equal(pt.traces(md * 1.05).cpu().numpy(), d_obs.cpu().numpy())
+    ss.clear_tuning()
+
+
 def test_fwi_problem_tune(ss, dev):
     """FwiProblem(tune=True) tunes both FWI kernels for its grid; its traces
     and gradient stay bit-identical to the global-memory kernels' (receiver
