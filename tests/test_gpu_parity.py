@@ -161,7 +161,8 @@ def test_naive_bit_exact_vs_oracle(ss, dev, oracle, shape):
 
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[2] % 4 == 0])
 @pytest.mark.parametrize("tx4,ty,n_chunks,xcd,nt", [(16, 16, 0, -1, -1), (16, 32, 1, 0, 1), (32, 8, 3, 1, 0),
-                                                   (32, 16, 5, 0, 1), (64, 4, 2, 1, 0), (64, 8, 0, -1, 1)])
+                                                   (32, 16, 5, 0, 1), (64, 4, 2, 1, 0), (64, 8, 0, -1, 1),
+                                                   (64, 12, 3, 0, 0), (64, 16, 2, 1, 1)])
 def test_tiled_exact_bit_exact_vs_oracle(ss, dev, oracle, shape, tx4, ty, n_chunks, xcd, nt):
     p0, vel2 = _inputs(oracle, shape, 12)
     grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
@@ -256,6 +257,33 @@ def test_heat_tiled_vs_oracle(ss, dev, oracle, exact):
         ss.heat_step(a, b, 1.0, grid, variant="naive")
         a, b = b, a
     assert np.array_equal(a.cpu().numpy(), ref)
+
+
+def test_heat_every_tile_shape_vs_oracle(ss, dev, oracle):
+    """The heat step at every built tile shape (1 to 3 z-chunks, both
+    workgroup orders and streaming modes) against the oracle: the reference
+    association bit-exact, the fused form within 1e-5 relative L2."""
+    import torch
+
+    shape = (33, 29, 56)
+    t0 = oracle.heat_noise_cube(shape, side=8, hot=10.0, seed=4)
+    dte = oracle.heat_dt_eff(np.float32(5.0))
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(10.0), np.float32(10.0), dte)
+    ref = oracle.heat_run(t0, np.float32(10.0), dte, np.float32(1.0), 3)
+    tiles = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
+    for j, (tx4, ty) in enumerate(tiles):
+        for chunks in (1, 2, 3):
+            kw = dict(tx4=tx4, ty=ty, n_chunks=chunks, xcd_remap=(j + chunks) % 2, nontemporal=(j + chunks + 1) % 2)
+            for exact in (True, False):
+                a, b = to_dev(t0, dev), torch.zeros(shape, device=dev)
+                for _ in range(3):
+                    ss.heat_step(a, b, 1.0, grid, exact=exact, **kw)
+                    a, b = b, a
+                out = a.cpu().numpy()
+                if exact:
+                    assert np.array_equal(out, ref), kw
+                else:
+                    assert oracle.rel_l2_interior(out, ref) <= 1e-5, kw
 
 
 @pytest.mark.parametrize("variant", ["naive", "tiled"])

@@ -113,6 +113,38 @@ def test_step16_fused_vs_oracle(ss, dev, oracle, fmt, shape, tile):
 
 
 @pytest.mark.parametrize("fmt", FORMATS)
+def test_step16_every_tile_shape_bit_exact(ss, dev, oracle, fmt):
+    """The 16-bit step in the reference association at every built tile shape
+    (1 to 3 z-chunks, both workgroup orders, both streaming modes) against the
+    oracle on a ragged grid, 3 steps (the second runs in the other sweep
+    direction)."""
+    shape = (29, 37, 48)
+    p0, vel2, ps, vs = _fields(ss, oracle, fmt, shape, dev, 57)
+    nz, ny, nx = shape
+    grid = ss.GridDims(nx, ny, nz, np.float32(10.0), np.float32(12.0), np.float32(8.0), np.float32(8e-4))
+    o = [oracle.encode16(fmt, p0, ps), oracle.encode16(fmt, p0, ps), oracle.encode16(fmt, np.zeros_like(p0), ps)]
+    ov = oracle.encode16(fmt, vel2, vs)
+    i = 0
+    for _ in range(3):
+        oracle.wave_step16(fmt, o[(i + 1) % 3], o[i], ov, o[(i + 2) % 3], grid.dx, grid.dt, ps, vs, grid.dy, grid.dz)
+        i = (i + 1) % 3
+    tiles = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
+    for j, (tx4, ty) in enumerate(tiles):
+        for chunks in (1, 2, 3):
+            shp = ss.tile_shape(tx4, ty, chunks, (j + chunks) % 2, (j + chunks + 1) % 2)
+            g = [ss.encode16(fmt, to_dev(a, dev), ss.Field16(fmt, shape, dev, ps)) for a in (p0, p0, np.zeros_like(p0))]
+            gv = ss.encode16(fmt, to_dev(vel2, dev), ss.Field16(fmt, shape, dev, vs))
+            i = 0
+            for _ in range(3):
+                ss.launch_tiled_16(g[(i + 1) % 3], g[i], gv, g[(i + 2) % 3], grid, tile=shp, exact=True)
+                i = (i + 1) % 3
+            for k in range(3):
+                assert np.array_equal(g[k].data.cpu().numpy().view(np.uint16), o[k][0]), (tx4, ty, chunks, k)
+                if fmt == "bfp16":
+                    assert np.array_equal(g[k].exp.cpu().numpy(), o[k][1]), (tx4, ty, chunks, k)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
 def test_tune16_caches_a_shape_with_identical_results(ss, dev, oracle, fmt):
     """The 16-bit autotuner caches its pick for NULL-shape launches on this
     grid and format; a NULL-shape launch then equals the explicit shape."""
