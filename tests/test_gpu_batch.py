@@ -1,7 +1,7 @@
 """Batched multi-shot forward modelling on the GPU: every shot of a ragged
-batch equals the oracle's single-shot run bit for bit, reruns of the captured
-graph are identical, tuned and default launch shapes agree, and bad shots are
-rejected."""
+batch equals the oracle's single-shot run bit for bit (default, tuned and every
+built launch shape), reruns of the captured graph are identical, and bad shots
+are rejected."""
 from __future__ import annotations
 
 import numpy as np
@@ -69,6 +69,30 @@ def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev):
     for i in range(len(shots)):
         assert bool((b2.newest(i) == first[i]).all()), i
     b2.close()
+    ss.clear_tuning()
+
+
+def test_batch_every_tile_shape_matches_oracle(ss, dev, oracle):
+    """The batched forward kernel at every built tile shape (1 to 3 z-chunks,
+    both workgroup orders), given to the groups through the tuner cache:
+    every shot equals the oracle's single-shot run bit for bit."""
+    n = 12
+    shots, host = _shots(ss, dev, n, seed=5)
+    want = [oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w) for m, w, src in host]
+    tiles = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
+    for j, (tx4, ty) in enumerate(tiles):
+        for chunks in (1, 3):
+            ss.clear_tuning()
+            shp = {"tile_x": 4 * tx4, "tile_y": ty, "n_chunks": chunks, "xcd_remap": (j + chunks) % 2,
+                   "nontemporal": 1}
+            for s in shots:
+                ss.set_tuned_shape(s["grid"], "wave", shp)
+            b = ss.ShotBatch(shots, n, dev, tune=False)
+            assert all(g["shape"] == shp for g in b.groups())
+            b.run()
+            for i in range(len(shots)):
+                assert np.array_equal(b.newest(i).cpu().numpy(), want[i]), (tx4, ty, chunks, i)
+            b.close()
     ss.clear_tuning()
 
 
