@@ -9,7 +9,11 @@
  * the reference layout: x fastest, idx = z*nx*ny + y*nx + x, dims include a
  * radius-4 halo on every face that the launchers never write
  * (include/stencil/types.hpp:110-136 of the reference).  p_next must not
- * alias an input.
+ * alias an input.  Beyond the launchers: the tuners synchronise, the batch
+ * object allocates its small pointer tables at create, and the FWI backward
+ * pass runs on a second stream of its own (created once per device) joined
+ * to the caller's by events it creates per call; device memory beyond the
+ * inputs and outputs is always the caller's (workspaces).
  */
 #ifndef STENCIL_CAPI_H
 #define STENCIL_CAPI_H
