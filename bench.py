@@ -528,9 +528,10 @@ def fwi_study(ss, n, steps, dev, tune=True):
     torch.cuda.synchronize()
     fwd_ms, bwd_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
     interior = (n - 8) ** 3
-    # forward 16 B/pt, recompute 16 + 4 (L p), adjoint 28 (lambda1, lambda2,
-    # m, L p, grad in; lambda, grad out) per interior point and step
-    alg = 64.0 * interior * steps
+    # per interior point and step: forward 16 B, recompute 16 + 4 (L p);
+    # adjoint 16 (psi1, psi2, m in; psi out) plus the gradient pass every
+    # other step (L p^k, L p^{k+1}, grad in; grad out: 16 B), 24 B on average
+    alg = 60.0 * interior * steps
     grad_ms = fwd_ms + bwd_ms
     out = {"grid": n, "steps": steps, "ckpt_every": pr.ckpt_every, "dtype": "f32",
            "grad_ms": round(grad_ms, 2), "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
@@ -543,7 +544,7 @@ def fwi_study(ss, n, steps, dev, tune=True):
     del pr
     torch.cuda.empty_cache()
     # One segment (ckpt_every = steps): every L p^k stays in HBM, so the
-    # backward pass is the adjoint steps alone (48 B/pt/step instead of 64).
+    # backward pass is the adjoint steps alone (44 B/pt/step instead of 60).
     cfg_all = dict(ckpt_every=steps)
     need = 1.05 * 4.0 * n ** 3 * (steps + 10)
     if torch.cuda.mem_get_info()[0] > need:

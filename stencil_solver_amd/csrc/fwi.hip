@@ -77,13 +77,17 @@ __global__ __launch_bounds__(kNX* kNY) void fwd_step_kernel(const T* __restrict_
 
 // psi_out = 2 psi1 - psi2 + dt^2 m L psi1  (+ m res[y][x] on plane rec_z);
 // grad += dt^2 lap * psi1 (lap = L p^k of the same k; the sum is divided by m
-// once at the end); trace[0] = psi_out / m at src (the adjoint field there).
+// once at the end), preceded, when lap2 = L p^{k+1} is given, by step k+1's
+// term dt^2 lap2 * psi2 (psi2 = psi^{k+2}): the two additions in the order and
+// rounding of the two steps applying one each, with one gradient read and
+// write; trace[0] = psi_out / m at src (the adjoint field there).
 template <class T>
 __global__ __launch_bounds__(kNX* kNY) void adj_step_kernel(const T* __restrict__ psi1, const T* __restrict__ psi2,
                                                             const T* __restrict__ m, const T* __restrict__ lap,
-                                                            T* __restrict__ grad, T* __restrict__ psi_out,
-                                                            const T* __restrict__ res, int rec_z, int nx, int ny,
-                                                            Scalars<T> s, long long src, T* __restrict__ trace) {
+                                                            const T* __restrict__ lap2, T* __restrict__ grad,
+                                                            T* __restrict__ psi_out, const T* __restrict__ res,
+                                                            int rec_z, int nx, int ny, Scalars<T> s, long long src,
+                                                            T* __restrict__ trace) {
     Cell k;
     if (!interior_cell(nx, ny, k)) return;
     const T pc = psi1[k.c];
@@ -92,10 +96,16 @@ __global__ __launch_bounds__(kNX* kNY) void adj_step_kernel(const T* __restrict_
     const T lz = lap_axis(psi1, k.c, k.sz, pc, s.inv_dz2);
     const T lp = lx + ly + lz;
     const T mc = m[k.c];
-    T v = T(2) * pc - psi2[k.c] + s.dt2 * mc * lp;
+    const T p2 = psi2[k.c];
+    T v = T(2) * pc - p2 + s.dt2 * mc * lp;
     if (res && k.iz == rec_z) v += mc * res[static_cast<size_t>(k.iy) * k.sy + k.ix];
     psi_out[k.c] = v;
-    if (lap) grad[k.c] += s.dt2 * lap[k.c] * pc;
+    if (lap) {
+        T g = grad[k.c];
+        if (lap2) g += s.dt2 * lap2[k.c] * p2;
+        g += s.dt2 * lap[k.c] * pc;
+        grad[k.c] = g;
+    }
     if (trace && static_cast<long long>(k.c) == src) trace[0] = v / mc;
 }
 
@@ -168,11 +178,13 @@ hipError_t fwd_step_t(const T* p, const T* pp, const T* m, T* pn, T* lap_out, co
 }
 
 template <class T>
-hipError_t adj_step_t(const T* psi1, const T* psi2, const T* m, const T* lap, T* grad, T* psi_out, const T* res,
-                      int rec_z, const Dims& d, const Scalars<T>& s, long long src, T* trace, hipStream_t st) {
+hipError_t adj_step_t(const T* psi1, const T* psi2, const T* m, const T* lap, const T* lap2, T* grad, T* psi_out,
+                      const T* res, int rec_z, const Dims& d, const Scalars<T>& s, long long src, T* trace,
+                      hipStream_t st) {
     if (d.nx <= 2 * kR || d.ny <= 2 * kR || d.nz <= 2 * kR) return hipSuccess;
-    hipLaunchKernelGGL(adj_step_kernel<T>, grid_of(d), dim3(kNX, kNY, 1), 0, st, psi1, psi2, m, lap, grad, psi_out,
-                       res, rec_z, static_cast<int>(d.nx), static_cast<int>(d.ny), s, src, trace);
+    if (lap2 && !lap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(adj_step_kernel<T>, grid_of(d), dim3(kNX, kNY, 1), 0, st, psi1, psi2, m, lap, lap2, grad,
+                       psi_out, res, rec_z, static_cast<int>(d.nx), static_cast<int>(d.ny), s, src, trace);
     return hipGetLastError();
 }
 
@@ -211,15 +223,15 @@ hipError_t fwi_fwd_step_f64(const double* p, const double* pp, const double* m, 
                             hipStream_t st) {
     return fwd_step_t(p, pp, m, pn, lap_out, d, s, src, wavelet, n, st);
 }
-hipError_t fwi_adj_step_f32(const float* psi1, const float* psi2, const float* m, const float* lap, float* grad,
-                            float* psi_out, const float* res, int rec_z, const Dims& d, const Scalars<float>& s,
-                            long long src, float* trace, hipStream_t st) {
-    return adj_step_t(psi1, psi2, m, lap, grad, psi_out, res, rec_z, d, s, src, trace, st);
+hipError_t fwi_adj_step_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
+                            float* grad, float* psi_out, const float* res, int rec_z, const Dims& d,
+                            const Scalars<float>& s, long long src, float* trace, hipStream_t st) {
+    return adj_step_t(psi1, psi2, m, lap, lap2, grad, psi_out, res, rec_z, d, s, src, trace, st);
 }
 hipError_t fwi_adj_step_f64(const double* psi1, const double* psi2, const double* m, const double* lap,
-                            double* grad, double* psi_out, const double* res, int rec_z, const Dims& d,
-                            const Scalars<double>& s, long long src, double* trace, hipStream_t st) {
-    return adj_step_t(psi1, psi2, m, lap, grad, psi_out, res, rec_z, d, s, src, trace, st);
+                            const double* lap2, double* grad, double* psi_out, const double* res, int rec_z,
+                            const Dims& d, const Scalars<double>& s, long long src, double* trace, hipStream_t st) {
+    return adj_step_t(psi1, psi2, m, lap, lap2, grad, psi_out, res, rec_z, d, s, src, trace, st);
 }
 hipError_t fwi_divide_f32(float* g, const float* m, const Dims& d, hipStream_t st) { return divide_t(g, m, d, st); }
 hipError_t fwi_divide_f64(double* g, const double* m, const Dims& d, hipStream_t st) { return divide_t(g, m, d, st); }

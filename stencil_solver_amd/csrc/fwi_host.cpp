@@ -37,10 +37,10 @@ hipError_t fwi_fwd_step_f32(const float*, const float*, const float*, float*, fl
                             const Scalars<float>&, long long, const float*, int, hipStream_t);
 hipError_t fwi_fwd_step_f64(const double*, const double*, const double*, double*, double*, const Dims&,
                             const Scalars<double>&, long long, const double*, int, hipStream_t);
-hipError_t fwi_adj_step_f32(const float*, const float*, const float*, const float*, float*, float*, const float*,
-                            int, const Dims&, const Scalars<float>&, long long, float*, hipStream_t);
-hipError_t fwi_adj_step_f64(const double*, const double*, const double*, const double*, double*, double*,
-                            const double*, int, const Dims&, const Scalars<double>&, long long, double*,
+hipError_t fwi_adj_step_f32(const float*, const float*, const float*, const float*, const float*, float*, float*,
+                            const float*, int, const Dims&, const Scalars<float>&, long long, float*, hipStream_t);
+hipError_t fwi_adj_step_f64(const double*, const double*, const double*, const double*, const double*, double*,
+                            double*, const double*, int, const Dims&, const Scalars<double>&, long long, double*,
                             hipStream_t);
 hipError_t fwi_residual_f32(const float*, const float*, float*, size_t, double*, double*, hipStream_t);
 hipError_t fwi_residual_f64(const double*, const double*, double*, size_t, double*, double*, hipStream_t);
@@ -162,15 +162,16 @@ struct Ops<float> {
         const size_t plane = size_t(c.d.nx) * size_t(c.d.ny);
         return hipMemcpyAsync(rec, pn + size_t(rec_z) * plane, plane * sizeof(float), hipMemcpyDeviceToDevice, st);
     }
-    static hipError_t adj(const float* l1, const float* l2, const float* m, const float* lap, float* g, float* lo,
-                          const float* res, int rz, const StepCtx& c, const Scalars<float>& s, float* tr,
-                          hipStream_t st) {
+    // lap2 (nullable, with lap): L p^{k+1}, step k+1's gradient term added first
+    static hipError_t adj(const float* l1, const float* l2, const float* m, const float* lap, const float* lap2,
+                          float* g, float* lo, const float* res, int rz, const StepCtx& c, const Scalars<float>& s,
+                          float* tr, hipStream_t st) {
         const void* ptrs[4] = {l1, l2, m, lo};
         if (c.tiled && zq_supported(c.d, ptrs, 4) && (!lap || (aligned16(lap) && aligned16(g))) &&
-            (!res || aligned16(res)))
-            return adjoint_zq_f32(l1, l2, m, lap, g, lo, res, rz, c.src_z, c.src_y, c.src_x, tr, c.d, s,
+            (!lap2 || aligned16(lap2)) && (!res || aligned16(res)))
+            return adjoint_zq_f32(l1, l2, m, lap, lap2, g, lo, res, rz, c.src_z, c.src_y, c.src_x, tr, c.d, s,
                                   c.adj_shape, st);
-        return fwi_adj_step_f32(l1, l2, m, lap, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
+        return fwi_adj_step_f32(l1, l2, m, lap, lap2, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
     }
     static hipError_t residual(const float* a, const float* b, float* r, size_t n, double* part, double* out,
                                hipStream_t st) {
@@ -191,10 +192,10 @@ struct Ops<double> {
         const size_t plane = size_t(c.d.nx) * size_t(c.d.ny);
         return hipMemcpyAsync(rec, pn + size_t(rec_z) * plane, plane * sizeof(double), hipMemcpyDeviceToDevice, st);
     }
-    static hipError_t adj(const double* l1, const double* l2, const double* m, const double* lap, double* g,
-                          double* lo, const double* res, int rz, const StepCtx& c, const Scalars<double>& s,
-                          double* tr, hipStream_t st) {
-        return fwi_adj_step_f64(l1, l2, m, lap, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
+    static hipError_t adj(const double* l1, const double* l2, const double* m, const double* lap,
+                          const double* lap2, double* g, double* lo, const double* res, int rz, const StepCtx& c,
+                          const Scalars<double>& s, double* tr, hipStream_t st) {
+        return fwi_adj_step_f64(l1, l2, m, lap, lap2, g, lo, res, rz, c.d, s, tr ? c.src : -1LL, tr, st);
     }
     static hipError_t residual(const double* a, const double* b, double* r, size_t n, double* part, double* out,
                                hipStream_t st) {
@@ -353,15 +354,27 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
         }
         return e;
     };
-    // The adjoint steps k = k_hi .. k_lo of segment sg (the k = N step skips
-    // the gradient term).
+    // The adjoint steps k = k_hi .. k_lo of segment sg.  Gradient terms
+    // T_k = dt^2 (L p^k) psi^{k+1} exist for k in [a, b] = [k_lo, min(k_hi,
+    // N-1)] (L p^N would meet lambda^{N+1} = 0).  Counting from the top of the
+    // segment, every other step defers its term to the step below, which adds
+    // T_{k+1} (it holds psi^{k+2} as its psi2) and then T_k: one gradient read
+    // and write per two steps, the same additions in the same order, and
+    // never a pair across segments (whose L p live in different banks).
     int l1 = 0, l2 = 1, lo = 2;  // psi^{k+1}, psi^{k+2}, psi^k (psi = m lambda)
     auto adjoint = [&](int64_t sg) -> hipError_t {
         hipError_t e = hipSuccess;
+        const int64_t a = k_lo(sg), b = std::min(k_hi(sg), p.n - 1);
         for (int64_t k = k_hi(sg); e == hipSuccess && k >= k_lo(sg); --k) {
-            const T* lap = k < p.n ? lap_slot(sg, k - k_lo(sg)) : nullptr;
-            e = Ops<T>::adj(lam[l1], lam[l2], m, lap, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z, cx, s,
-                            src_trace ? src_trace + (k - 1) : nullptr, st);
+            const T* lap = nullptr;
+            const T* lap2 = nullptr;
+            if (k <= b) {
+                const bool defer = (b - k) % 2 == 0 && k > a;
+                if (!defer) lap = lap_slot(sg, k - a);
+                if ((b - k) % 2 == 1) lap2 = lap_slot(sg, k + 1 - a);
+            }
+            e = Ops<T>::adj(lam[l1], lam[l2], m, lap, lap2, grad, lam[lo], res + size_t(k - 1) * p.plane, p.rec_z,
+                            cx, s, src_trace ? src_trace + (k - 1) : nullptr, st);
             const int t = l2;
             l2 = l1;
             l1 = lo;
@@ -438,9 +451,12 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     for (int i = 0; i < 3; ++i) f.push_back(at<float>(work, p.ring + size_t(i) * field));
     for (int i = 0; i < 3; ++i) f.push_back(at<float>(work, p.lam + size_t(i) * field));
     if (p.banks == 2)
-        for (int64_t i = 0; i < std::min<int64_t>(p.banks * p.C, 4); ++i)
+        for (int64_t i = 0; i < std::min<int64_t>(p.banks * p.C, 5); ++i)
             f.push_back(at<float>(work, p.lap + size_t(i) * field));
     const bool pair = f.size() >= 10;
+    // the adjoint's second L p field: a field of its own where there is one
+    const float* lap2_solo = f.size() >= 7 ? f[6] : f[0];
+    const float* lap2_pair = f.size() >= 11 ? f[10] : f[0];
     float* trace = at<float>(work, p.partial);  // reduction scratch
     const void* ptrs[4] = {f[0], f[1], f[2], f[3]};
     if (!zq_supported(p.d, ptrs, 4)) return STENCIL_ERR_UNSUPPORTED;
@@ -465,9 +481,14 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
         ++ch.i;
         return e;
     };
-    auto adj = [&](Chain& ch, const ZqShape& sh, const float* lap, float* grad, const float* res) {
-        const hipError_t e = adjoint_zq_f32(ch.cur(), ch.prev(), m, lap, grad, ch.next(), res, p.rec_z, cx.src_z,
-                                            cx.src_y, cx.src_x, trace, p.d, s, sh, st);
+    // adjoint steps alternate as the backward pass runs them: one adding two
+    // gradient terms (L p^k, L p^{k+1}), the next none
+    auto adj = [&](Chain& ch, const ZqShape& sh, const float* lap, const float* lap2, float* grad,
+                   const float* res) {
+        const bool pair = ch.i % 2 == 0;
+        const hipError_t e = adjoint_zq_f32(ch.cur(), ch.prev(), m, pair ? lap : nullptr, pair ? lap2 : nullptr,
+                                            grad, ch.next(), res, p.rec_z, cx.src_z, cx.src_y, cx.src_x, trace, p.d,
+                                            s, sh, st);
         ++ch.i;
         return e;
     };
@@ -484,7 +505,7 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
         // the adjoint runs 64 x 4 and 64 x 16 as 64 x 8: not timed twice
         if (!(sh.txv == 64 && (sh.ty == 4 || sh.ty == 16))) {
             Chain b{{f[1], f[2], f[3]}};
-            r.adj_ms = zq_time_launches([&] { return adj(b, sh, f[4], f[5], f[4]); }, reps, st);
+            r.adj_ms = zq_time_launches([&] { return adj(b, sh, f[4], lap2_solo, f[5], f[4]); }, reps, st);
         }
         rows.push_back(r);
     }
@@ -518,7 +539,8 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
                     if (e == hipSuccess) e = hipStreamWaitEvent(rs, fork, 0);
                     for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = fwd(a, r.shape, f[4], nullptr, rs);
                     if (e == hipSuccess) e = hipEventRecord(join, rs);
-                    for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = adj(b, adj_best, f[5], f[9], f[5]);
+                    for (int k = 0; e == hipSuccess && k < kSteps; ++k)
+                        e = adj(b, adj_best, f[5], lap2_pair, f[9], f[5]);
                     if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
                     return e;
                 },
@@ -632,7 +654,7 @@ int adj_step_api(const T* l1, const T* l2, const T* m, const T* lap, T* grad, T*
     if (trace && (src < 0 || src >= cells)) return STENCIL_ERR_INVALID;
     StepCtx cx = step_ctx(dims_of_grid(g), trace ? (long long)src : -1LL);
     cx.tiled = variant != STENCIL_VARIANT_NAIVE;
-    return Ops<T>::adj(l1, l2, m, lap, grad, lo, res, int(rec_z), cx, scalars<T>(g), trace,
+    return Ops<T>::adj(l1, l2, m, lap, nullptr, grad, lo, res, int(rec_z), cx, scalars<T>(g), trace,
                        static_cast<hipStream_t>(stream)) == hipSuccess
                ? STENCIL_OK
                : STENCIL_ERR_HIP;

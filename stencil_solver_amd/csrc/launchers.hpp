@@ -98,13 +98,15 @@ hipError_t wave_zq_16(int fmt, const Field16& cur, const Field16& prev, const Fi
 // into lap_out when non-null, the interior of plane rec_z of p_next into
 // rec_plane (nx*ny, halo left as is) when non-null.  Adjoint, in psi = m lambda:
 // psi_out = 2 psi1 - psi2 + dt^2 m L psi1 (+ m res on plane rec_z when res is
-// non-null); grad += dt^2 lap psi1 when lap is non-null; *trace = psi_out / m
-// at the source cell when trace is non-null.
+// non-null); grad += dt^2 lap psi1 when lap is non-null, preceded by
+// grad += dt^2 lap2 psi2 when lap2 (L p^{k+1}, needs lap) is non-null: the
+// gradient terms of two steps in one pass, added as the two steps would;
+// *trace = psi_out / m at the source cell when trace is non-null.
 hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, float* pn, float* lap_out,
                            int src_z, int src_y, int src_x, const float* src_val, float* rec_plane, int rec_z,
                            const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st);
-hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, float* grad,
-                          float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
+                          float* grad, float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
                           float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st);
 // One shot of a batched forward launch (device table entry): three time-level
 // buffers, vel2, the source cell and its wavelet (nullable: no source).

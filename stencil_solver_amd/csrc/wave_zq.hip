@@ -71,10 +71,13 @@ struct ZqArgs {
     // nx*ny receiver plane that receives p^{n+1} on plane rec_z.  kAdjoint, in
     // psi = m lambda: cur = psi^{k+1}, prev = psi^{k+2}, vel2 = m, aux = L p^k
     // (nullable: no gradient term), grad += dt^2 (L p^k) psi^{k+1} in place,
-    // next = psi^k = 2 psi^{k+1} - psi^{k+2} + dt^2 m L psi^{k+1}, plus m times
-    // the residual plane res (nullable) on plane rec_z, trace (nullable) =
-    // psi^k / m at the source cell.
+    // preceded when aux2 = L p^{k+1} is non-null by step k+1's term
+    // dt^2 (L p^{k+1}) psi^{k+2} (the two additions as the two steps would
+    // make them, one gradient pass for both); next = psi^k = 2 psi^{k+1} -
+    // psi^{k+2} + dt^2 m L psi^{k+1}, plus m times the residual plane res
+    // (nullable) on plane rec_z, trace (nullable) = psi^k / m at the source cell.
     const void* aux;
+    const void* aux2;
     void* lap_out;
     void* grad;
     const float* res;
@@ -374,6 +377,7 @@ void zq_kernel(ZqArgs a) {
     const Rsrc vel_r = field(a.vel2, plane_bytes);
     const Rsrc next_r = field(a.next, plane_bytes);
     const Rsrc aux_r = field(a.aux, plane_bytes);
+    const Rsrc aux2_r = field(a.aux2, plane_bytes);
     const Rsrc grad_r = field(a.grad, plane_bytes);
     const Rsrc lap_r = field(a.lap_out, plane_bytes);
     const Rsrc cur_er = field(a.cur_e, eplane);
@@ -463,7 +467,7 @@ void zq_kernel(ZqArgs a) {
     // Steps past z_end (the tail of the last group of 9 in a chunk whose length
     // is not a multiple of 9) run on clamped planes and store nothing.
     auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 edge, float4 pv, float4 vv, float dt2v,
-                      float4 lc, float4 gr) {
+                      float4 lc, float4 gr, float4 l2) {
         // Every lane computes (its inputs are loaded, from offset 0 outside
         // the grid); only output lanes store, and block-FP16 leaves the other
         // lanes out of its block maximum.
@@ -492,8 +496,12 @@ void zq_kernel(ZqArgs a) {
                     }
                     if (a.aux) {  // gradient: grad += dt^2 * (L p^k) * psi^{k+1}
                         const float4 c = zq[kR];
-                        const float4 g = make_float4(gr.x + a.s.dt2 * lc.x * c.x, gr.y + a.s.dt2 * lc.y * c.y,
-                                                     gr.z + a.s.dt2 * lc.z * c.z, gr.w + a.s.dt2 * lc.w * c.w);
+                        float4 g = gr;
+                        if (a.aux2)  // first step k+1's term, dt^2 * (L p^{k+1}) * psi^{k+2}
+                            g = make_float4(g.x + a.s.dt2 * l2.x * pv.x, g.y + a.s.dt2 * l2.y * pv.y,
+                                            g.z + a.s.dt2 * l2.z * pv.z, g.w + a.s.dt2 * l2.w * pv.w);
+                        g = make_float4(g.x + a.s.dt2 * lc.x * c.x, g.y + a.s.dt2 * lc.y * c.y,
+                                        g.z + a.s.dt2 * lc.z * c.z, g.w + a.s.dt2 * lc.w * c.w);
                         store4<FMT, NT, true>(grad_r, soff(zz, plane_bytes), grad_r, 0, own_b, 0, g, 1.f,
                                               own_out, false);
                     }
@@ -525,18 +533,19 @@ void zq_kernel(ZqArgs a) {
         constexpr int kPD = prefetch_depth(TXV, TY, MODE);
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const bool lap_in = kAdj && a.aux != nullptr;
+        const bool lap2_in = lap_in && a.aux2 != nullptr;
         float4 rq[kPD], xeq[kPD];
         float4 hq[kPD][kHaloPer];
-        float4 pvq[kPD], vvq[kPD], lcq[kPD], grq[kPD];
+        float4 pvq[kPD], vvq[kPD], lcq[kPD], grq[kPD], l2q[kPD];
         // Loads of plane z into FIFO stage d (queue refill: plane z + kR).
         auto fetch = [&](int z, float4& r, float4& xe, float4 (&h)[kHaloPer], float4& pv, float4& vv, float4& lc,
-                         float4& gr) {
+                         float4& gr, float4& l2) {
             r = raw_at(cur_r, min(z + kR, z_last), own_ld);
             xe = zero4;
             if constexpr (kShuffleX) xe = raw_at(cur_r, z, xe_ld);
 #pragma unroll
             for (int k = 0; k < kHaloPer; ++k) h[k] = raw_at(cur_r, z, h_ld[k]);
-            pv = vv = lc = gr = zero4;
+            pv = vv = lc = gr = l2 = zero4;
             if constexpr (!kHeat) {
                 pv = raw_stream(prev_r, z);
                 vv = raw_stream(vel_r, z);
@@ -546,11 +555,12 @@ void zq_kernel(ZqArgs a) {
                     lc = raw_stream(aux_r, z);
                     gr = raw_stream(grad_r, z);
                 }
+                if (lap2_in) l2 = raw_stream(aux2_r, z);
             }
         };
 #pragma unroll
         for (int d = 0; d < kPD; ++d)
-            fetch(min(z_begin + d, z_last), rq[d], xeq[d], hq[d], pvq[d], vvq[d], lcq[d], grq[d]);
+            fetch(min(z_begin + d, z_last), rq[d], xeq[d], hq[d], pvq[d], vvq[d], lcq[d], grq[d], l2q[d]);
         for (int zb = z_begin; zb < z_end; zb += 9) {
             // The 9 steps are expanded at compile time (j is a constant in
             // every queue index), so the queue arrays stay in registers.
@@ -558,8 +568,8 @@ void zq_kernel(ZqArgs a) {
                 constexpr int j = decltype(jc)::value;
                 const int zz = zb + j;
                 q[(j + 8) % 9] = rq[0];
-                float4 r_n, xe_n, h_n[kHaloPer], pv_n, vv_n, lc_n, gr_n;
-                fetch(min(zz + kPD, z_last), r_n, xe_n, h_n, pv_n, vv_n, lc_n, gr_n);
+                float4 r_n, xe_n, h_n[kHaloPer], pv_n, vv_n, lc_n, gr_n, l2_n;
+                fetch(min(zz + kPD, z_last), r_n, xe_n, h_n, pv_n, vv_n, lc_n, gr_n, l2_n);
                 const int b = (zz - z_begin) & 1;
                 lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
@@ -569,7 +579,7 @@ void zq_kernel(ZqArgs a) {
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, xeq[0], pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0]);
+                finish(zz, b, zq, xeq[0], pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0], l2q[0]);
                 // FIFO shift (register renaming under the unroll)
 #pragma unroll
                 for (int d = 0; d + 1 < kPD; ++d) {
@@ -581,6 +591,7 @@ void zq_kernel(ZqArgs a) {
                     vvq[d] = vvq[d + 1];
                     lcq[d] = lcq[d + 1];
                     grq[d] = grq[d + 1];
+                    l2q[d] = l2q[d + 1];
                 }
                 rq[kPD - 1] = r_n;
                 xeq[kPD - 1] = xe_n;
@@ -590,6 +601,7 @@ void zq_kernel(ZqArgs a) {
                 vvq[kPD - 1] = vv_n;
                 lcq[kPD - 1] = lc_n;
                 grq[kPD - 1] = gr_n;
+                l2q[kPD - 1] = l2_n;
             });
         }
     } else {
@@ -657,7 +669,7 @@ void zq_kernel(ZqArgs a) {
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv, pv, vv, dt2v, pv, pv);
+                finish(zz, b, zq, pv, pv, vv, dt2v, pv, pv, pv);
             }
         }
     }
@@ -948,14 +960,16 @@ hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int st
     return launch_zq(a, d, shape, kWaveFwi, st, dev::kF32, n_shots);
 }
 
-hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, float* grad,
-                          float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
+                          float* grad, float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
                           float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+    if (lap2 && !lap) return hipErrorInvalidValue;
     ZqArgs a{};
     a.cur = psi1;
     a.prev = psi2;
     a.vel2 = m;
     a.aux = lap;
+    a.aux2 = lap2;
     a.grad = grad;
     a.next = psi_out;
     a.res = res;
