@@ -113,8 +113,9 @@ double stencil_heat_bytes_per_step(const stencil_grid* grid, int bytes_per_value
 
 /* Launch shape of the tiled kernel (autotuner / measurement).  Zero or -1
  * fields are chosen by the library.  (tile_x4, tile_y) must be one of the
- * built shapes: (16,16) (16,32) (32,8) (32,16) (64,4) (64,8) -- a tile is
- * 4*tile_x4 cells wide and tile_y rows tall, one thread per 4 cells. */
+ * built shapes: (16,16) (16,32) (32,8) (32,16) (64,4) (64,8) (64,12)
+ * (64,16) -- a tile is 4*tile_x4 cells wide and tile_y rows tall, one
+ * thread per 4 cells. */
 typedef struct stencil_tile_shape {
     int tile_x4;     /* float4 columns per tile row (0 = default) */
     int tile_y;      /* rows per tile (0 = default) */
@@ -260,6 +261,8 @@ int stencil_host_noise_cube_f32(float* host, uint64_t nx, uint64_t ny, uint64_t 
  * second stream of its own (created once per device, default priority),
  * overlapped with the adjoint steps of segment s on hip_stream and joined to
  * it by events: the call stays asynchronous and its work ends on hip_stream.
+ * With STENCIL_FWI_OVERLAP=0 in the environment everything runs on
+ * hip_stream, recompute then adjoint steps per segment (same results).
  * All device memory is the caller's workspace.  Arrays: vel2 and grad
  * nx*ny*nz, wavelet n_steps (device), traces / d_obs n_steps*ny*nx with trace
  * n holding p^{n+1} on plane rec_z. ---- */

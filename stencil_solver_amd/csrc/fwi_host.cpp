@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -271,6 +272,16 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
     return STENCIL_OK;
 }
 
+// STENCIL_FWI_OVERLAP=0 runs each segment's recompute on the caller's stream
+// before its adjoint steps (same results, one stream); read once.
+bool fwi_overlap() {
+    static const bool on = [] {
+        const char* e = std::getenv("STENCIL_FWI_OVERLAP");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // The recompute stream of the current device: created once (default
 // priority, non-blocking) and kept for the process, so a backward call
 // neither creates nor destroys streams; calls on one device share it.
@@ -385,6 +396,12 @@ int backward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, con
     if (p.S == 1) {
         // one segment: the misfit forward wrote every L p^k (bank 0)
         FWI_TRY(adjoint(0));
+    } else if (!fwi_overlap()) {
+        FWI_TRY(Ops<T>::fill(ring[2], p.cells, st));  // the zero walls, as below
+        for (int64_t sg = p.S - 1; sg >= 0; --sg) {
+            FWI_TRY(recompute(sg, st));
+            FWI_TRY(adjoint(sg));
+        }
     } else {
         // Segments last to first; the recompute of segment sg-1 runs on a
         // second stream while the adjoint steps of segment sg run on the

@@ -130,7 +130,10 @@ def main(argv=None):
     krow = next((r for r in stats if r["name"] == kname), None)
     import shlex
 
-    wrapper = "python -m stencil_solver_amd.profile " + " ".join(shlex.quote(x) for x in argv)
+    # the library's own environment switches are part of the command
+    envs = " ".join(f"{k}={shlex.quote(v)}" for k, v in sorted(os.environ.items()) if k.startswith("STENCIL_"))
+    wrapper = (envs + " " if envs else "") + "python -m stencil_solver_amd.profile " + " ".join(
+        shlex.quote(x) for x in argv)
     lines = [f"# rocprofv3 summary: {a.name}", "",
              "Made on one MI355X from the repository root by", "", "```", wrapper, "```", "",
              "which ran these passes:", "", "```"]
