@@ -502,8 +502,8 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     // gradient terms (L p^k, L p^{k+1}), the next none
     auto adj = [&](Chain& ch, const ZqShape& sh, const float* lap, const float* lap2, float* grad,
                    const float* res) {
-        const bool pair = ch.i % 2 == 0;
-        const hipError_t e = adjoint_zq_f32(ch.cur(), ch.prev(), m, pair ? lap : nullptr, pair ? lap2 : nullptr,
+        const bool terms = ch.i % 2 == 0;
+        const hipError_t e = adjoint_zq_f32(ch.cur(), ch.prev(), m, terms ? lap : nullptr, terms ? lap2 : nullptr,
                                             grad, ch.next(), res, p.rec_z, cx.src_z, cx.src_y, cx.src_x, trace, p.d,
                                             s, sh, st);
         ++ch.i;
@@ -537,9 +537,9 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     const Row* br = nullptr;
     if (pair) {
         // recompute chain f[1..3] (L p into f[4]) on the recompute stream, the
-        // adjoint chain f[6..8] (L p from f[5], gradient f[9]) on st, joined
-        // every kSteps steps (the backward pass joins once per segment)
-        // (a fork and a join event per timed call, each recorded once)
+        // adjoint chain f[6..8] (L p from f[5] and lap2_pair, gradient f[9])
+        // on st, joined every kSteps steps (the backward pass joins once per
+        // segment); a fork and a join event per timed call, each recorded once
         constexpr int kSteps = 8;
         for (Row& r : rows) {
             Chain a{{f[1], f[2], f[3]}};
