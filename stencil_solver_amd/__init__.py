@@ -40,4 +40,4 @@ from .api import (  # noqa: F401
 )
 from ._lib import LIB_PATH, lib  # noqa: F401
 from .batch import ShotBatch  # noqa: F401
-from .fwi import FwiProblem, fwi_misfit, ricker  # noqa: F401
+from .fwi import FwiProblem, adjoint_step, forward_step, fwi_misfit, ricker  # noqa: F401

@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from ._lib import check, lib, stencil_fwi_config, stencil_tile_shape
-from .api import GridDims, _shape_dict, _stream
+from .api import GridDims, _dev_ptr, _shape_dict, _stream, _variant
 
 
 def ricker(n_steps: int, dt: float, f_peak: float, t0: float | None = None, amplitude: float = 1.0) -> np.ndarray:
@@ -146,6 +146,55 @@ class FwiProblem:
         return (grad, tr) if src_trace else grad
 
 
+def _sfx_of(t) -> str:
+    import torch
+
+    if t.dtype == torch.float32:
+        return "f32"
+    if t.dtype == torch.float64:
+        return "f64"
+    raise TypeError("fields must be float32 or float64")
+
+
+def forward_step(p_cur, p_prev, vel2, p_next, grid: GridDims, lap_out=None, src_index: int = -1, wavelet=None,
+                 n: int = 0, variant: str = "tiled", stream=None) -> None:
+    """One forward FWI step (stencil_forward_step_*): p_next = 2 p_cur - p_prev
+    + dt^2 vel2 L p_cur in the reference association, plus wavelet[n] at the
+    linear cell src_index (< 0: no source); L p_cur into lap_out when given."""
+    sfx = _sfx_of(p_cur)
+    dt = p_cur.dtype
+    ptr = [_dev_ptr(t, grid, dt, nm) for t, nm in ((p_cur, "p_cur"), (p_prev, "p_prev"), (vel2, "vel2"),
+                                                       (p_next, "p_next"))]
+    lp = _dev_ptr(lap_out, grid, dt, "lap_out") if lap_out is not None else None
+    w = wavelet.data_ptr() if wavelet is not None else None
+    g = grid.to_c()
+    check(getattr(lib(), f"stencil_forward_step_{sfx}")(*ptr, lp, C.byref(g), int(src_index), w, int(n),
+                                                        _variant(variant), _stream(stream)), "forward_step")
+
+
+def adjoint_step(psi1, psi2, vel2, psi_out, grid: GridDims, lap=None, grad=None, res_plane=None, rec_z: int = 0,
+                 src_index: int = -1, src_value=None, variant: str = "tiled", stream=None) -> None:
+    """One adjoint step in psi = vel2 * lambda (stencil_adjoint_step_*):
+    psi_out = 2 psi1 - psi2 + dt^2 vel2 L psi1 (+ vel2 * res_plane on plane
+    rec_z), grad += dt^2 lap psi1 when lap is given, src_value[0] =
+    psi_out / vel2 at src_index when given."""
+    sfx = _sfx_of(psi1)
+    dt = psi1.dtype
+    ptr = [_dev_ptr(t, grid, dt, nm) for t, nm in ((psi1, "psi1"), (psi2, "psi2"), (vel2, "vel2"))]
+    lp = _dev_ptr(lap, grid, dt, "lap") if lap is not None else None
+    gp = _dev_ptr(grad, grid, dt, "grad") if grad is not None else None
+    op = _dev_ptr(psi_out, grid, dt, "psi_out")
+    if res_plane is not None and (res_plane.dtype != dt or res_plane.numel() != grid.nx * grid.ny
+                                  or not res_plane.is_contiguous()):
+        raise ValueError(f"res_plane must be a contiguous {dt} tensor of nx*ny elements")
+    rp = res_plane.data_ptr() if res_plane is not None else None
+    sv = src_value.data_ptr() if src_value is not None else None
+    g = grid.to_c()
+    check(getattr(lib(), f"stencil_adjoint_step_{sfx}")(*ptr, lp, gp, op, rp, int(rec_z), C.byref(g),
+                                                        int(src_index), sv, _variant(variant), _stream(stream)),
+          "adjoint_step")
+
+
 def _autograd_function():
     import torch
 
@@ -180,4 +229,4 @@ def fwi_misfit(vel2, problem: FwiProblem, d_obs):
     return _FWI_FN.apply(vel2, problem, d_obs)
 
 
-__all__ = ["ricker", "FwiProblem", "fwi_misfit"]
+__all__ = ["ricker", "FwiProblem", "fwi_misfit", "forward_step", "adjoint_step"]

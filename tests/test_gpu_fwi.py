@@ -107,6 +107,66 @@ def test_tiled_fwi_equals_naive(ss, dev, shape, tuned):
     ss.clear_tuning()
 
 
+@pytest.mark.parametrize("variant", ["naive", "tiled"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_single_steps_match_oracle_step(ss, dev, oracle, dtype, variant):
+    """stencil_forward_step / stencil_adjoint_step against the oracle's wave
+    step plus the FWI terms formed in numpy in the same precision and order:
+    the forward step with and without the source, L p out (the oracle step
+    with p_prev = 2p, dt = 0.5 and vel2 = 4 returns L p exactly), and the
+    adjoint step with the residual plane, the gradient term and the source
+    value."""
+    import torch
+
+    shape = (26, 33, 44)
+    nz, ny, nx = shape
+    h, dt = dtype(10.0), dtype(1e-3)
+    rng = np.random.default_rng(9)
+    halo0 = np.zeros(shape, dtype)
+    halo0[4:-4, 4:-4, 4:-4] = 1
+    a, b = (rng.standard_normal(shape).astype(dtype) * halo0 for _ in range(2))
+    m = (4e6 * (1 + 0.1 * rng.random(shape))).astype(dtype)
+    grid = ss.GridDims(nx, ny, nz, h, h, h, dt)
+    src = (13, 16, 21)
+    si = (src[0] * ny + src[1]) * nx + src[2]
+    w = np.array([0.25, -1.5, 3.0], dtype)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+    # forward: oracle step, then the source term
+    want = oracle.wave_step(a, b, m, np.zeros(shape, dtype), h, dt)
+    out, lap = d(np.zeros(shape, dtype)), d(np.zeros(shape, dtype))
+    ss.forward_step(d(a), d(b), d(m), out, grid, variant=variant)
+    assert np.array_equal(out.cpu().numpy(), want)
+    want_src = want.copy()
+    want_src.flat[si] = want.flat[si] + w[2]
+    ss.forward_step(d(a), d(b), d(m), out, grid, lap_out=lap, src_index=si, wavelet=d(w), n=2, variant=variant)
+    assert np.array_equal(out.cpu().numpy(), want_src)
+    # L a, exactly: 2a - 2a = 0, then (0.25 * 4) * L a
+    lap_want = oracle.wave_step(a, 2 * a, np.full(shape, 4, dtype), np.zeros(shape, dtype), h, dtype(0.5))
+    assert np.array_equal(lap.cpu().numpy(), lap_want) and np.abs(lap_want).max() > 0
+
+    # adjoint: the same step in psi, then + m * res on plane rec_z, the
+    # gradient term and psi_out / m at the source
+    rec_z = 6
+    res = rng.standard_normal((ny, nx)).astype(dtype)
+    g0 = rng.standard_normal(shape).astype(dtype)
+    base = oracle.wave_step(a, b, m, np.zeros(shape, dtype), h, dt)
+    psi_want = base.copy()
+    plane = psi_want[rec_z]
+    plane[4:-4, 4:-4] = plane[4:-4, 4:-4] + m[rec_z, 4:-4, 4:-4] * res[4:-4, 4:-4]
+    g_want = g0.copy()
+    g_want[4:-4, 4:-4, 4:-4] = g0[4:-4, 4:-4, 4:-4] + (dt * dt * lap_want[4:-4, 4:-4, 4:-4]) * a[4:-4, 4:-4, 4:-4]
+    psi, grad, sv = d(np.zeros(shape, dtype)), d(g0), d(np.zeros(2, dtype))
+    ss.adjoint_step(d(a), d(b), d(m), psi, grid, lap=d(lap_want), grad=grad, res_plane=d(res), rec_z=rec_z,
+                    src_index=si, src_value=sv, variant=variant)
+    assert np.array_equal(psi.cpu().numpy(), psi_want)
+    assert np.array_equal(grad.cpu().numpy(), g_want)
+    assert sv.cpu().numpy()[0] == psi_want.flat[si] / m.flat[si]
+    # no optional inputs: the plain step
+    ss.adjoint_step(d(a), d(b), d(m), psi, grid, variant=variant)
+    assert np.array_equal(psi.cpu().numpy(), base)
+
+
 @pytest.mark.parametrize("shape", [(30, 44, 72), (41, 20, 36)])
 def test_fwi_every_tile_shape_equals_naive(ss, dev, shape):
     """The FP32 FWI forward, recompute and adjoint kernels at every built tile
