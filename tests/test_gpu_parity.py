@@ -259,6 +259,29 @@ def test_heat_tiled_vs_oracle(ss, dev, oracle, exact):
     assert np.array_equal(a.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("variant", ["naive", "tiled"])
+def test_heat_fp64_is_the_wave_step_with_alpha(ss, dev, oracle, variant):
+    """The FP64 heat step equals the reference's definition of it, the wave
+    step with vel2 = alpha and p_prev = p_cur (heat_diffusion.cpp:103-122),
+    as the oracle's FP64 wave step computes it, bit for bit over 4 steps."""
+    import torch
+
+    shape = (30, 27, 41)
+    rng = np.random.default_rng(8)
+    t = np.zeros(shape)
+    t[4:-4, 4:-4, 4:-4] = rng.random((22, 19, 33))
+    alpha, dte = 1.5, float(np.sqrt(0.05 * 100.0 / 1.5))
+    grid = ss.GridDims(shape[2], shape[1], shape[0], 10.0, 10.0, 10.0, dte)
+    ref = t.copy()
+    for _ in range(4):
+        ref = oracle.wave_step(ref, ref, np.full(shape, alpha), np.zeros(shape), 10.0, dte)
+    a, b = to_dev(t, dev), torch.zeros(shape, dtype=torch.float64, device=dev)
+    for _ in range(4):
+        ss.heat_step(a, b, alpha, grid, variant=variant)
+        a, b = b, a
+    assert np.array_equal(a.cpu().numpy(), ref) and np.abs(ref - t).max() > 0
+
+
 def test_heat_every_tile_shape_vs_oracle(ss, dev, oracle):
     """The heat step at every built tile shape (1 to 3 z-chunks, both
     workgroup orders and streaming modes) against the oracle: the reference
