@@ -173,3 +173,20 @@ def test_set_tuned_shape_round_trip_and_validation():
         ss.set_tuned_shape(g, "wave", dict(shp, xcd_remap=2))
     ss.clear_tuning()
     assert not ss.tuned_shape(g, "fwi_adjoint")[1]
+
+
+def test_hip_errors_are_reported_not_raised():
+    """A HIP failure comes back as STENCIL_ERR_HIP with the runtime's code
+    kept for stencil_last_hip_error (no exit, no exception); with no device
+    that is the coefficient upload's device query (hipErrorNoDevice)."""
+    import torch
+
+    import stencil_solver_amd as ss
+
+    L = ss.lib()
+    r = L.stencil_upload_fd_coefficients()
+    if torch.cuda.device_count() == 0:
+        assert r == 2 and L.stencil_strerror(r) == b"HIP runtime error"
+        assert L.stencil_last_hip_error() == 100  # hipErrorNoDevice
+    else:
+        assert r == 0
