@@ -208,6 +208,30 @@ def test_fwi_every_tile_shape_equals_naive(ss, dev, shape):
     ss.clear_tuning()
 
 
+@pytest.mark.parametrize("ckpt", [1, 24])
+def test_fwi_tune_without_overlap_timing(ss, dev, ckpt):
+    """With one checkpoint per step, or one segment (every L p kept), the FWI
+    tuner has no room or no use for the recompute-beside-adjoint timing: the
+    recompute shape repeats the forward's and the gradient still equals the
+    global-memory kernels' bit for bit."""
+    import torch
+
+    shape, n = (36, 40, 104), 24
+    m = smooth_model(shape, 4, 1800.0, 3000.0, np.float32)
+    w = (1e3 * ss.ricker(n, 1e-3, 20.0)).astype(np.float32)
+    grid = _grid(ss, shape, 10.0, 1e-3, np.float32)
+    md = torch.from_numpy(m).to(dev)
+    ss.clear_tuning()
+    pt = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, ckpt, tune=True)
+    assert pt.shape["recompute"] == pt.shape["forward"]
+    assert not ss.tuned_shape(grid, "fwi_recompute")[1]
+    pn = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, ckpt, "naive")
+    d_obs = pn.traces(md * 1.05)
+    assert pt.misfit(md, d_obs).item() == pn.misfit(md, d_obs).item()
+    assert np.array_equal(pt.gradient(md).cpu().numpy(), pn.gradient(md).cpu().numpy())
+    ss.clear_tuning()
+
+
 def test_fwi_problem_tune(ss, dev):
     """FwiProblem(tune=True) tunes both FWI kernels for its grid; its traces
     and gradient stay bit-identical to the global-memory kernels' (receiver
