@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include <sys/mman.h>
 #include <unistd.h>
 
 namespace mt {
@@ -89,8 +90,7 @@ inline bool double_eq(double a, double b) {
 // Captures what a callable writes to stdout.
 inline std::string capture_stdout(const std::function<void()>& fn) {
     std::fflush(stdout);
-    char path[] = "/tmp/mt_stdout_XXXXXX";
-    const int fd = mkstemp(path);
+    const int fd = memfd_create("mt_stdout", 0);  // anonymous in-memory file
     const int saved = dup(1);
     dup2(fd, 1);
     fn();
@@ -103,7 +103,6 @@ inline std::string capture_stdout(const std::function<void()>& fn) {
     ssize_t n;
     while ((n = read(fd, buf, sizeof(buf))) > 0) out.append(buf, static_cast<size_t>(n));
     close(fd);
-    unlink(path);
     return out;
 }
 
