@@ -515,7 +515,6 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     };
     std::vector<Row> rows;
     for (const ZqShape& sh : zq_tune_candidates(p.d)) {
-        if (sh.nt != 1) continue;
         Row r{sh, -1.f, -1.f, -1.f};
         Chain a{{f[1], f[2], f[3]}};
         r.fwd_ms = zq_time_launches([&] { return fwd(a, sh, nullptr, f[5], st); }, reps, st);

@@ -24,7 +24,9 @@
 //     z+5 and the stream and halo loads for plane z+1 go out before plane z is
 //     computed (gfx950 barriers do not drain outstanding global loads).  With
 //     NT they use non-temporal accesses so they do not displace p_cur lines
-//     in L2.
+//     in L2; without, all four fields can stay in the 256 MB memory-side
+//     cache on grids that fit it (256^3: 4 x 67 MB).  The FWI forward step's
+//     L p output is non-temporal either way (written once, read much later).
 // HBM traffic per interior point and step: read p_cur, p_prev, vel2 once and
 // write p_next once = 16 B (measured 1.05x: halo re-reads are L2 hits, the
 // tile order keeping neighbouring tiles on one XCD).  Chained steps alternate
@@ -485,9 +487,9 @@ void zq_kernel(ZqArgs a) {
                     if (a.trace && zz == a.rec_z)
                         store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), 0, plane_rsrc(a.trace, 0, 0),
                                                  0, own_b, 0, out, 1.f, own_out, false);
-                    if (a.lap_out)
-                        store4<FMT, NT, true>(lap_r, soff(zz, plane_bytes), lap_r, 0, own_b, 0, lap4, 1.f,
-                                              own_out, false);
+                    if (a.lap_out)  // written once, read much later: non-temporal in either mode
+                        store4<FMT, true, true>(lap_r, soff(zz, plane_bytes), lap_r, 0, own_b, 0, lap4, 1.f,
+                                                own_out, false);
                 } else {
                     if (a.res && zz == a.rec_z && own_out) {  // + m * residual
                         const float4 r = *reinterpret_cast<const float4*>(a.res + own_off);
@@ -700,8 +702,8 @@ KernelFn pick_shape(int txv, int ty) {
 
 KernelFn pick(int txv, int ty, int mode, bool nt) {
     switch (mode) {
-        case kWaveFwi: return pick_shape<kWaveFwi, true>(txv, ty);  // FWI modes: non-temporal streams only
-        case kAdjoint: return pick_shape<kAdjoint, true>(txv, ty);
+        case kWaveFwi: return nt ? pick_shape<kWaveFwi, true>(txv, ty) : pick_shape<kWaveFwi, false>(txv, ty);
+        case kAdjoint: return nt ? pick_shape<kAdjoint, true>(txv, ty) : pick_shape<kAdjoint, false>(txv, ty);
         case kWaveFused: return nt ? pick_shape<kWaveFused, true>(txv, ty) : pick_shape<kWaveFused, false>(txv, ty);
         case kWaveExact: return nt ? pick_shape<kWaveExact, true>(txv, ty) : pick_shape<kWaveExact, false>(txv, ty);
         case kHeatFused: return nt ? pick_shape<kHeatFused, true>(txv, ty) : pick_shape<kHeatFused, false>(txv, ty);
@@ -945,7 +947,7 @@ hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, floa
     a.src_y = src_y;
     a.src_x = src_x;
     a.s = s;
-    shape.nt = 1;
+    if (shape.nt < 0) shape.nt = 1;
     return launch_zq(a, d, shape, kWaveFwi, st);
 }
 
@@ -979,7 +981,7 @@ hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, 
     a.src_y = src_y;
     a.src_x = src_x;
     a.s = s;
-    shape.nt = 1;
+    if (shape.nt < 0) shape.nt = 1;
     return launch_zq(a, d, zq_resolve(d, shape, kKindFwiAdjoint), kAdjoint, st);
 }
 
