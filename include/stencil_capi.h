@@ -296,7 +296,7 @@ int stencil_fwi_backward_f64(const double* vel2, const stencil_grid* grid, const
                              const double* wavelet, double* grad, double* src_trace, void* work,
                              void* hip_stream);
 /* Autotuner of the FP32 FWI kernels (no reference counterpart): the sweep of
- * stencil_tune_tiled_f32 (non-temporal streams only) applied to the forward
+ * stencil_tune_tiled_f32 (both streaming modes) applied to the forward
  * step as the forward pass runs it (recording the receiver plane), to the
  * adjoint step, and to the forward step writing L p while the chosen adjoint
  * step runs beside it on the second stream, as the backward pass recomputes

@@ -450,8 +450,9 @@ void put_shape(const ZqShape& z, stencil_tile_shape* out) {
 
 // Launch-shape sweep of the FP32 FWI kernels on the workspace's wavefield,
 // adjoint and (with more than one segment) Laplacian buffers, over the
-// tuner's shapes with non-temporal streams (the only mode these kernels run
-// in).  Each timed chain rotates three buffers as the leapfrog loops do, so
+// tuner's shapes and both streaming modes (cached accesses win where a
+// step's fields fit the memory-side cache).  Each timed chain rotates three
+// buffers as the leapfrog loops do, so
 // the sweep-direction alternation between consecutive steps is part of what
 // is timed: the forward step recording the receiver plane (the forward pass),
 // the adjoint step alone, and the forward step writing L p while the best
