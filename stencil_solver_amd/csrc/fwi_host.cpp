@@ -455,8 +455,9 @@ void put_shape(const ZqShape& z, stencil_tile_shape* out) {
 // buffers as the leapfrog loops do, so
 // the sweep-direction alternation between consecutive steps is part of what
 // is timed: the forward step recording the receiver plane (the forward pass),
-// the adjoint step alone, and the forward step writing L p while the best
-// adjoint step runs beside it on the recompute stream (the backward pass).
+// the adjoint step alone, and the forward step writing L p while the fastest
+// adjoint step of either streaming mode runs beside it on the recompute
+// stream (the backward pass, which takes the best such pair).
 int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int reps, stencil_tile_shape* best_fwd,
          stencil_tile_shape* best_adj, stencil_tile_shape* best_rec, const char* csv_path, hipStream_t st) {
     Plan p;
@@ -512,11 +513,11 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     };
     struct Row {
         ZqShape shape;
-        float fwd_ms, adj_ms, rec_ms;
+        float fwd_ms, adj_ms, rec_ms[2];  // rec_ms[k]: beside the best adjoint shape with nt == k
     };
     std::vector<Row> rows;
     for (const ZqShape& sh : zq_tune_candidates(p.d)) {
-        Row r{sh, -1.f, -1.f, -1.f};
+        Row r{sh, -1.f, -1.f, {-1.f, -1.f}};
         Chain a{{f[1], f[2], f[3]}};
         r.fwd_ms = zq_time_launches([&] { return fwd(a, sh, nullptr, f[5], st); }, reps, st);
         // the adjoint runs 64 x 4 and 64 x 16 as 64 x 8: not timed twice
@@ -528,44 +529,59 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     }
     const Row* bf = nullptr;
     const Row* ba = nullptr;
+    const Row* ba_nt[2] = {nullptr, nullptr};  // the fastest adjoint in each streaming mode
     for (const Row& r : rows) {
         if (r.fwd_ms > 0.f && (!bf || r.fwd_ms < bf->fwd_ms)) bf = &r;
         if (r.adj_ms > 0.f && (!ba || r.adj_ms < ba->adj_ms)) ba = &r;
+        const int k = r.shape.nt ? 1 : 0;
+        if (r.adj_ms > 0.f && (!ba_nt[k] || r.adj_ms < ba_nt[k]->adj_ms)) ba_nt[k] = &r;
     }
     if (!bf || !ba) return STENCIL_ERR_HIP;
-    const ZqShape adj_best = ba->shape;
+    ZqShape adj_best = ba->shape;  // alone; with the overlap, the best pair's
     const Row* br = nullptr;
+    int br_k = 0;
     if (pair) {
         // recompute chain f[1..3] (L p into f[4]) on the recompute stream, the
         // adjoint chain f[6..8] (L p from f[5] and lap2_pair, gradient f[9])
         // on st, joined every kSteps steps (the backward pass joins once per
         // segment); a fork and a join event per timed call, each recorded once
+        // Both adjoint modes are tried: a cached adjoint that wins alone can
+        // lose beside a recompute competing for the same memory-side cache.
         constexpr int kSteps = 8;
-        for (Row& r : rows) {
-            Chain a{{f[1], f[2], f[3]}};
-            Chain b{{f[6], f[7], f[8]}};
-            Events evs;
-            FWI_TRY(evs.init(size_t(2 * (reps + 2))));
-            size_t call = 0;
-            const float ms = zq_time_launches(
-                [&] {
-                    if (2 * call + 1 >= evs.ev.size()) return hipErrorInvalidValue;  // more calls than events
-                    hipEvent_t fork = evs.ev[2 * call], join = evs.ev[2 * call + 1];
-                    ++call;
-                    hipError_t e = hipEventRecord(fork, st);
-                    if (e == hipSuccess) e = hipStreamWaitEvent(rs, fork, 0);
-                    for (int k = 0; e == hipSuccess && k < kSteps; ++k) e = fwd(a, r.shape, f[4], nullptr, rs);
-                    if (e == hipSuccess) e = hipEventRecord(join, rs);
-                    for (int k = 0; e == hipSuccess && k < kSteps; ++k)
-                        e = adj(b, adj_best, f[5], lap2_pair, f[9], f[5]);
-                    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
-                    return e;
-                },
-                reps, st);
-            if (ms > 0.f) r.rec_ms = ms / kSteps;
+        for (int k = 0; k < 2; ++k) {
+            if (!ba_nt[k]) continue;
+            const ZqShape adj_k = ba_nt[k]->shape;
+            for (Row& r : rows) {
+                Chain a{{f[1], f[2], f[3]}};
+                Chain b{{f[6], f[7], f[8]}};
+                Events evs;
+                FWI_TRY(evs.init(size_t(2 * (reps + 2))));
+                size_t call = 0;
+                const float ms = zq_time_launches(
+                    [&] {
+                        if (2 * call + 1 >= evs.ev.size()) return hipErrorInvalidValue;  // more calls than events
+                        hipEvent_t fork = evs.ev[2 * call], join = evs.ev[2 * call + 1];
+                        ++call;
+                        hipError_t e = hipEventRecord(fork, st);
+                        if (e == hipSuccess) e = hipStreamWaitEvent(rs, fork, 0);
+                        for (int j = 0; e == hipSuccess && j < kSteps; ++j) e = fwd(a, r.shape, f[4], nullptr, rs);
+                        if (e == hipSuccess) e = hipEventRecord(join, rs);
+                        for (int j = 0; e == hipSuccess && j < kSteps; ++j)
+                            e = adj(b, adj_k, f[5], lap2_pair, f[9], f[5]);
+                        if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+                        return e;
+                    },
+                    reps, st);
+                if (ms > 0.f) r.rec_ms[k] = ms / kSteps;
+            }
         }
-        for (const Row& r : rows)
-            if (r.rec_ms > 0.f && (!br || r.rec_ms < br->rec_ms)) br = &r;
+        for (int k = 0; k < 2; ++k)
+            for (const Row& r : rows)
+                if (r.rec_ms[k] > 0.f && (!br || r.rec_ms[k] < br->rec_ms[br_k])) {
+                    br = &r;
+                    br_k = k;
+                }
+        if (br) adj_best = ba_nt[br_k]->shape;
     }
     zq_cache_put(p.d, kKindFwiForward, bf->shape);
     zq_cache_put(p.d, kKindFwiAdjoint, adj_best);
@@ -576,15 +592,16 @@ int tune(const stencil_grid* g, const stencil_fwi_config* c, void* work, int rep
     if (csv_path && *csv_path) {
         FILE* out = std::fopen(csv_path, "w");
         if (!out) return STENCIL_ERR_INVALID;
-        std::fprintf(out, "kernel,nx,ny,nz,tile_x,tile_y,n_chunks,xcd_remap,ms\n");
-        static const char* const names[3] = {"forward", "adjoint", "recompute_beside_adjoint"};
+        std::fprintf(out, "kernel,nx,ny,nz,tile_x,tile_y,n_chunks,xcd_remap,nontemporal,ms\n");
+        static const char* const names[4] = {"forward", "adjoint", "recompute_beside_cached_adjoint",
+                                             "recompute_beside_nontemporal_adjoint"};
         for (const Row& r : rows)
-            for (int k = 0; k < 3; ++k) {
-                const float ms = k == 0 ? r.fwd_ms : k == 1 ? r.adj_ms : r.rec_ms;
+            for (int k = 0; k < 4; ++k) {
+                const float ms = k == 0 ? r.fwd_ms : k == 1 ? r.adj_ms : r.rec_ms[k - 2];
                 if (ms > 0.f)
-                    std::fprintf(out, "%s,%lld,%lld,%lld,%d,%d,%d,%d,%.5f\n", names[k], (long long)p.d.nx,
+                    std::fprintf(out, "%s,%lld,%lld,%lld,%d,%d,%d,%d,%d,%.5f\n", names[k], (long long)p.d.nx,
                                  (long long)p.d.ny, (long long)p.d.nz, 4 * r.shape.txv, r.shape.ty, r.shape.n_chunks,
-                                 r.shape.xcd_remap, ms);
+                                 r.shape.xcd_remap, r.shape.nt, ms);
             }
         std::fclose(out);
     }

@@ -365,6 +365,11 @@ void zq_kernel(ZqArgs a) {
     auto raw_stream = [plane_bytes, own_ld, soff](Rsrc f, int z) -> Raw<FMT> {
         return load_raw<FMT, NT>(f, own_ld, soff(z, plane_bytes));
     };
+    // Fields read once per pass (the adjoint's L p inputs): non-temporal in
+    // either mode, like the forward step's L p output.
+    auto raw_once = [plane_bytes, own_ld, soff](Rsrc f, int z) -> Raw<FMT> {
+        return load_raw<FMT, true>(f, own_ld, soff(z, plane_bytes));
+    };
     auto exp_at = [eplane, soff](Rsrc e, int z, unsigned eoff) -> int {
         (void)eplane;
         (void)soff;
@@ -554,10 +559,10 @@ void zq_kernel(ZqArgs a) {
             }
             if constexpr (kAdj) {
                 if (lap_in) {
-                    lc = raw_stream(aux_r, z);
+                    lc = raw_once(aux_r, z);
                     gr = raw_stream(grad_r, z);
                 }
-                if (lap2_in) l2 = raw_stream(aux2_r, z);
+                if (lap2_in) l2 = raw_once(aux2_r, z);
             }
         };
 #pragma unroll
