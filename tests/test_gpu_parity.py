@@ -309,6 +309,49 @@ def test_heat_every_tile_shape_vs_oracle(ss, dev, oracle):
                     assert oracle.rel_l2_interior(out, ref) <= 1e-5, kw
 
 
+def test_tiled_steps_replay_in_a_graph(ss, dev, oracle):
+    """Ten tiled steps captured in a HIP graph (torch.cuda.CUDAGraph) and
+    replayed three times, with eager steps and synchronisations between the
+    replays: every replay equals the oracle's ten steps bit for bit (the
+    launchers record no memset and allocate nothing)."""
+    import torch
+
+    shape = (40, 36, 48)
+    p0, vel2 = _inputs(oracle, shape, 5)
+    grid = ss.GridDims(shape[2], shape[1], shape[0], np.float32(10.0), np.float32(12.0), np.float32(8.0),
+                       np.float32(8e-4))
+    want = oracle.wave_run(p0, vel2, grid.dx, grid.dt, 10, grid.dy, grid.dz)
+    v = to_dev(vel2, dev)
+    init = [to_dev(p0, dev), to_dev(p0, dev), torch.zeros(shape, device=dev)]
+
+    def steps(buf):  # p^{n+1} into buf[(n+2) % 3]; after 10 steps the newest is buf[2]
+        for i in range(10):
+            ss.launch_tiled(buf[(i + 1) % 3], buf[i % 3], v, buf[(i + 2) % 3], grid)
+
+    bufs = [t.clone() for t in init]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):  # warm-up outside the capture
+        steps(bufs)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        steps(bufs)
+    other = [t.clone() for t in init]
+    for _ in range(3):
+        for k in range(3):
+            bufs[k].copy_(init[k])
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(bufs[2].cpu().numpy(), want)
+        for k in range(3):  # eager steps between the replays
+            other[k].copy_(init[k])
+        steps(other)
+        torch.cuda.synchronize()
+        assert np.array_equal(other[2].cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("variant", ["naive", "tiled"])
 def test_halo_never_written(ss, dev, oracle, variant):
     import torch
