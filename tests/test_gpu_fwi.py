@@ -233,6 +233,43 @@ def test_fwi_tune_without_overlap_timing(ss, dev, ckpt):
     ss.clear_tuning()
 
 
+def test_gradient_replays_in_a_graph(ss, dev):
+    """misfit() + gradient() captured in a HIP graph (torch.cuda.CUDAGraph):
+    the backward pass's second stream joins the capture through its events,
+    and three replays, with eager gradients between them, give the eager
+    gradient bit for bit."""
+    import torch
+
+    shape, n = (36, 40, 104), 24
+    m = smooth_model(shape, 4, 1800.0, 3000.0, np.float32)
+    w = (1e3 * ss.ricker(n, 1e-3, 20.0)).astype(np.float32)
+    grid = _grid(ss, shape, 10.0, 1e-3, np.float32)
+    md = torch.from_numpy(m).to(dev)
+    pr = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32)
+    d_obs = pr.traces(md * 1.05)
+    pr.misfit(md, d_obs)
+    want = pr.gradient(md).cpu().numpy()
+    assert np.abs(want).max() > 0
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):  # warm-up outside the capture
+        pr.misfit(md, d_obs)
+        pr.gradient(md)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    out = torch.empty_like(md)
+    with torch.cuda.graph(g):
+        pr.misfit(md, d_obs)
+        out.copy_(pr.gradient(md))
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want)
+        pr.misfit(md, d_obs)
+        assert np.array_equal(pr.gradient(md).cpu().numpy(), want)
+
+
 def test_fwi_problem_tune(ss, dev):
     """FwiProblem(tune=True) tunes both FWI kernels for its grid; its traces
     and gradient stay bit-identical to the global-memory kernels' (receiver
