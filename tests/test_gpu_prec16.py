@@ -145,6 +145,60 @@ def test_step16_every_tile_shape_bit_exact(ss, dev, oracle, fmt):
 
 
 @pytest.mark.parametrize("fmt", FORMATS)
+def test_step16_replays_in_a_graph(ss, dev, oracle, fmt):
+    """Six fused 16-bit steps captured in a HIP graph replay, three times with
+    eager steps between, to the eager steps' stored bits (values and block
+    exponents)."""
+    import torch
+
+    shape = (28, 36, 64)
+    p0, vel2, ps, vs = _fields(ss, oracle, fmt, shape, dev, 44)
+    nz, ny, nx = shape
+    grid = ss.GridDims(nx, ny, nz, np.float32(10.0), np.float32(12.0), np.float32(8.0), np.float32(8e-4))
+    gv = ss.encode16(fmt, to_dev(vel2, dev), ss.Field16(fmt, shape, dev, vs))
+
+    def fresh():
+        return [ss.encode16(fmt, to_dev(a, dev), ss.Field16(fmt, shape, dev, ps)) for a in (p0, p0, np.zeros_like(p0))]
+
+    def steps(f):  # the newest level ends in f[6 % 3 + 1] = f[1]
+        for i in range(6):
+            ss.launch_tiled_16(f[(i + 1) % 3], f[i % 3], gv, f[(i + 2) % 3], grid)
+
+    def bits(f):
+        return f.data.cpu().numpy().copy(), (f.exp.cpu().numpy().copy() if fmt == "bfp16" else None)
+
+    ref = fresh()
+    steps(ref)
+    torch.cuda.synchronize()
+    want = bits(ref[1])
+    static, init = fresh(), fresh()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):  # warm-up outside the capture
+        steps(static)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        steps(static)
+    for _ in range(3):
+        for k in range(3):
+            static[k].data.copy_(init[k].data)
+            if fmt == "bfp16":
+                static[k].exp.copy_(init[k].exp)
+        g.replay()
+        torch.cuda.synchronize()
+        got = bits(static[1])
+        assert np.array_equal(got[0], want[0])
+        if fmt == "bfp16":
+            assert np.array_equal(got[1], want[1])
+        other = fresh()
+        steps(other)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(other[1])[0], want[0])
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
 def test_tune16_caches_a_shape_with_identical_results(ss, dev, oracle, fmt):
     """The 16-bit autotuner caches its pick for NULL-shape launches on this
     grid and format; a NULL-shape launch then equals the explicit shape."""
