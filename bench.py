@@ -62,8 +62,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path has no halo exchange here: one GPU, one grid).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the step runs one grid on one GPU, with no halo exchange).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
