@@ -1,7 +1,7 @@
 """CPU oracle for the 8th-order stencil path (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
-import this package; it is the checker, never the thing measured or shipped.
+import this package; it checks results and is never the thing measured or shipped.
 
 * ``Oracle``     -- ctypes binding of liboracle.so, the C restatement of the
                     reference CPU path (oracle/stencil_oracle.c).

@@ -1,7 +1,7 @@
 /*
  * oracle/stencil_oracle.c -- CPU restatement of the reference stencil path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load this library; the product never
  * does.  Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  *

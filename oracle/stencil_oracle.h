@@ -1,7 +1,7 @@
 /*
  * oracle/stencil_oracle.h -- CPU restatement of the reference stencil path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library checks results: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product (stencil_solver_amd/, include/) never links or calls it.
  *
