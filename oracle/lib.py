@@ -269,6 +269,7 @@ class RefLib:
         L.ref_solver.argtypes = [C.c_int, C.c_int] + [_SZ] * 3 + [T] * 7 + [C.c_int] * 3 + [_P]
         L.ref_solver.restype = _D
         L.ref_init_gaussian.argtypes = [_P] + [_SZ] * 3 + [T] * 4
+        L.ref_init_layered_velocity.argtypes = [_P] + [_SZ] * 3 + [T] * 2
         L.ref_make_grid_dt.argtypes = [T] * 4
         L.ref_make_grid_dt.restype = T
         L.ref_l2_error.argtypes = [_P, _P] + [_SZ] * 3
@@ -292,6 +293,12 @@ class RefLib:
     def gaussian(self, n, cx=0.5, cy=0.5, cz=0.5, sigma=5.0):
         out = np.empty((n, n, n), dtype=self.dtype)
         self.lib.ref_init_gaussian(_ptr(out), n, n, n, cx, cy, cz, sigma)
+        return out
+
+    def layered_velocity(self, shape, v_top, v_bot):
+        nz, ny, nx = shape
+        out = np.empty(shape, dtype=self.dtype)
+        self.lib.ref_init_layered_velocity(_ptr(out), nx, ny, nz, v_top, v_bot)
         return out
 
     def make_grid_dt(self, dx, velocity):

@@ -81,6 +81,18 @@ void ref_init_gaussian(real_t* out, size_t nx, size_t ny, size_t nz, real_t cx, 
     std::memcpy(out, h.ptr(), h.bytes());
 }
 
+// init_layered_velocity (include/stencil/grid.hpp:91-99).
+void ref_init_layered_velocity(real_t* out, size_t nx, size_t ny, size_t nz, real_t v_top,
+                               real_t v_bot) {
+    GridDims g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    HostGrid h(g);
+    init_layered_velocity(h, v_top, v_bot);
+    std::memcpy(out, h.ptr(), h.bytes());
+}
+
 // Config::make_grid().dt (include/stencil/config.hpp:75).
 real_t ref_make_grid_dt(real_t dx, real_t dy, real_t dz, real_t velocity) {
     Config cfg;

@@ -132,6 +132,24 @@ def main():
         meta["gaussian"].append({"n": n, "sigma": sg, "c": c, "sha": sha(gg),
                                  "centre_val": float(gg[n // 2, n // 2, n // 2])})
 
+    # 6. init_layered_velocity (grid.hpp:91-99): even and odd nz (the split is
+    #    at nz / 2 in integer division), every cell including the halo.
+    meta["layered_velocity"] = []
+    for shape, vt, vb in (((16, 8, 8), 1500.0, 3000.0), ((17, 6, 5), 2000.0, 4500.0), ((9, 4, 3), 1234.5, 987.25)):
+        lv = nofma.layered_velocity(shape, vt, vb)
+        meta["layered_velocity"].append({"shape": list(shape), "v_top": vt, "v_bot": vb, "sha": sha(lv),
+                                         "first": float(lv.reshape(-1)[0]), "last": float(lv.reshape(-1)[-1])})
+
+    # 7. l2_error / max_abs_error (grid.cpp:86-115) on seeded inputs that the
+    #    tests regenerate (numpy default_rng(seed), U[-1, 1), float32).
+    meta["norms"] = []
+    for seed, shape in ((31, (9, 10, 11)), (32, (12, 12, 12))):
+        rng = np.random.default_rng(seed)
+        a = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+        b = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+        meta["norms"].append({"seed": seed, "shape": list(shape), "l2_error": nofma.l2_error(a, b),
+                              "max_abs_error": nofma.max_abs_error(a, b)})
+
     (OUT / "golden_meta.json").write_text(json.dumps(meta, indent=1, sort_keys=True))
     print("wrote", sorted(p.name for p in OUT.glob("*.npz")))
 

@@ -10,6 +10,11 @@
 3. Each closed-form check is shown to fail on a deliberately wrong restatement
    (a planted mistake in a numpy copy of the step, on the CPU), so the checks
    are not restatements of the formula they test.
+4. The relative-L2 function every tolerance test uses (it has no reference
+   counterpart) against values its definition fixes (a == b, a == 2b, a == 0,
+   halo cells ignored, one perturbed cell on each interior edge, b == 0), with
+   planted mistakes (halo counted, off-by-one interior, no square root, wrong
+   denominator, RMS, transposed axes, b narrowed to float32) each caught.
 """
 from __future__ import annotations
 
@@ -134,6 +139,31 @@ def test_acoustic128_fp64_matches_reference(oracle, golden_meta):
     fin = oracle.wave_run(p0, vel2, 10.0, 1e-3, 200)
     assert sha(p0) == golden_meta["acoustic128_f64"]["sha_init"]
     assert sha(fin) == golden_meta["acoustic128_f64"]["sha_final"]
+
+
+def test_layered_velocity_matches_reference(oracle, golden_meta):
+    """init_layered_velocity (grid.hpp:91-99) against the reference's own
+    output: v_top^2 for iz < nz/2, v_bot^2 below, in every cell."""
+    for row in golden_meta["layered_velocity"]:
+        shape = tuple(row["shape"])
+        lv = oracle.layered_velocity(shape, np.float32(row["v_top"]), np.float32(row["v_bot"]))
+        assert sha(lv) == row["sha"]
+        assert float(lv.reshape(-1)[0]) == row["first"] and float(lv.reshape(-1)[-1]) == row["last"]
+        vt2 = np.float32(row["v_top"]) * np.float32(row["v_top"])
+        vb2 = np.float32(row["v_bot"]) * np.float32(row["v_bot"])
+        nz = shape[0]
+        assert np.all(lv[: nz // 2] == vt2) and np.all(lv[nz // 2:] == vb2)
+
+
+def test_norms_match_reference(oracle, golden_meta):
+    """l2_error (absolute RMS over every cell) and max_abs_error against the
+    reference's outputs on the same seeded inputs."""
+    for row in golden_meta["norms"]:
+        rng = np.random.default_rng(row["seed"])
+        a = rng.uniform(-1.0, 1.0, tuple(row["shape"])).astype(np.float32)
+        b = rng.uniform(-1.0, 1.0, tuple(row["shape"])).astype(np.float32)
+        assert oracle.l2_error(a, b) == row["l2_error"]
+        assert oracle.max_abs_error(a, b) == row["max_abs_error"]
 
 
 def test_norms_match_reference_semantics(oracle):
@@ -310,3 +340,90 @@ def test_planted_mistakes_are_caught(mutation):
     bad = _numpy_step(mutation)
     caught = check_polynomial_exactness(bad) > 1e-6 or check_dispersion(bad) > 1e-6
     assert caught, mutation
+
+
+# ── 4. the relative-L2 checker itself (no reference counterpart: the reference
+#       has only absolute RMS, grid.cpp:86-97), pinned by closed forms ───────
+def check_rel_l2(fn, f64_ref=False):
+    """Largest deviation of fn(a, b) = ||a - b||_2 / ||b||_2 over the interior
+    [R, n-R)^3 from values the definition fixes; b is float64 when f64_ref."""
+    rng = np.random.default_rng(11)
+    shape = (10, 11, 13)  # nz, ny, nx all different: a transposed index shows
+    bt = np.float64 if f64_ref else np.float32
+    b = rng.uniform(0.5, 2.0, shape).astype(bt)
+    n_int = (shape[0] - 2 * R) * (shape[1] - 2 * R) * (shape[2] - 2 * R)
+    f32 = lambda x: np.array(x, dtype=np.float32)  # noqa: E731  (always a copy)
+    errs = []
+    errs.append(abs(fn(f32(b), b) - 0.0) if not f64_ref else 0.0)  # a == b
+    b2 = b.astype(np.float32).astype(bt)  # exactly representable in float32
+    errs.append(abs(fn(f32(b2), b2) - 0.0))
+    errs.append(abs(fn(f32(2 * b2), b2) - 1.0))  # a == 2b
+    errs.append(abs(fn(f32(np.zeros(shape)), b2) - 1.0))  # a == 0
+    errs.append(abs(fn(f32(-b2), b2) - 2.0))  # a == -b
+    # halo cells are ignored: scribble over every halo cell of a and b
+    a = f32(2 * b2)
+    ah, bh = a.copy(), b2.copy()
+    halo = np.ones(shape, bool)
+    halo[R:-R, R:-R, R:-R] = False
+    ah[halo] = 1e6
+    bh[halo] = -3e5
+    errs.append(abs(fn(ah, bh) - fn(a, b2)))
+    # one interior cell off by d, every interior b = 1: d / sqrt(N_interior);
+    # the cells chosen sit on the interior's last row / column / plane
+    ones = np.zeros(shape, bt)
+    ones[R:-R, R:-R, R:-R] = 1.0
+    for cell in ((shape[0] - R - 1, R, R), (R, shape[1] - R - 1, R), (R, R, shape[2] - R - 1)):
+        a1 = f32(ones)
+        a1[cell] += 0.5
+        errs.append(abs(fn(a1, ones) - 0.5 / np.sqrt(n_int)))
+    # the first halo cell past each interior edge does not count
+    for cell in ((shape[0] - R, R, R), (R, shape[1] - R, R), (R, R, shape[2] - R), (R - 1, R, R)):
+        a1 = f32(ones)
+        a1[cell] += 0.5
+        errs.append(abs(fn(a1, ones)))
+    # b == 0 on the interior: the absolute norm sqrt(sum a^2)
+    z = np.zeros(shape, bt)
+    a1 = np.zeros(shape, np.float32)
+    a1[R, R + 1, R + 2] = 3.0
+    a1[R + 1, R, R] = 4.0
+    errs.append(abs(fn(a1, z) - 5.0))
+    if f64_ref:
+        # b in double, not representable in float32: |fl32(1/3) - 1/3| / (1/3)
+        third = np.full(shape, 1.0 / 3.0)
+        want = abs(float(np.float32(1.0 / 3.0)) - 1.0 / 3.0) * 3.0
+        errs.append(abs(fn(f32(third), third) - want) / want * 1e-3)
+    return max(errs)
+
+
+def test_rel_l2_interior_closed_forms(oracle):
+    assert check_rel_l2(oracle.rel_l2_interior) < 1e-12
+    assert check_rel_l2(oracle.rel_l2_interior, f64_ref=True) < 1e-12
+
+
+def _numpy_rel_l2(mutation=None):
+    def fn(a, b):
+        lo = R + 1 if mutation == "off_by_one" else R
+        s = (slice(lo, -R),) * 3 if mutation != "halo_included" else (slice(None),) * 3
+        if mutation == "transposed":
+            s = (slice(R, -R), slice(R, -R), slice(R, a.shape[2] - R)) if a.shape[0] == a.shape[2] else \
+                (slice(R, a.shape[2] - R), slice(R, -R), slice(R, a.shape[0] - R))
+        bb = b[s].astype(np.float32 if mutation == "b_in_f32" else np.float64).astype(np.float64)
+        d = a[s].astype(np.float64) - bb
+        num = float((d * d).sum())
+        den = float((bb * bb).sum()) if mutation != "den_from_a" else float((a[s].astype(np.float64) ** 2).sum())
+        if mutation == "rms":
+            return np.sqrt(num / d.size)
+        if mutation == "no_sqrt":
+            return num / den if den > 0 else num
+        return np.sqrt(num / den) if den > 0 else np.sqrt(num)
+    return fn
+
+
+@pytest.mark.parametrize("mutation", ["halo_included", "off_by_one", "no_sqrt", "den_from_a", "rms",
+                                      "transposed", "b_in_f32"])
+def test_rel_l2_planted_mistakes_are_caught(mutation):
+    assert check_rel_l2(_numpy_rel_l2(None)) < 1e-12
+    assert check_rel_l2(_numpy_rel_l2(None), f64_ref=True) < 1e-12
+    bad = _numpy_rel_l2(mutation)
+    worst = max(check_rel_l2(bad), check_rel_l2(bad, f64_ref=True))
+    assert worst > 1e-9, mutation
