@@ -77,6 +77,12 @@ class FwiProblem:
         self.wavelet = w.to(self.device).contiguous()
         self.misfit_dev = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._have_residual = False
+        # Bumped by every call that overwrites the workspace's checkpoints and
+        # residual (misfit, traces): a gradient belongs to the misfit call of
+        # the same generation, and to the vel2 tensor (storage and version)
+        # that call saw.
+        self.generation = 0
+        self._misfit_vel2 = None
         self._sfx = "f32" if self.dtype == torch.float32 else "f64"
         self.shape = None
         if tune and self.dtype == torch.float32 and variant == "tiled":
@@ -117,7 +123,9 @@ class FwiProblem:
         fn = getattr(lib(), f"stencil_fwi_forward_{self._sfx}")
         check(fn(vel2.data_ptr(), C.byref(self._g), C.byref(self.cfg), self.wavelet.data_ptr(), None,
                  out.data_ptr(), None, self.work.data_ptr(), _stream(stream)), "fwi_forward")
+        self.generation += 1
         self._have_residual = False
+        self._misfit_vel2 = None
         return out
 
     def misfit(self, vel2, d_obs, stream=None):
@@ -128,15 +136,21 @@ class FwiProblem:
         fn = getattr(lib(), f"stencil_fwi_forward_{self._sfx}")
         check(fn(vel2.data_ptr(), C.byref(self._g), C.byref(self.cfg), self.wavelet.data_ptr(), d_obs.data_ptr(),
                  None, self.misfit_dev.data_ptr(), self.work.data_ptr(), _stream(stream)), "fwi_forward")
+        self.generation += 1
         self._have_residual = True
+        self._misfit_vel2 = (vel2.data_ptr(), vel2._version)
         return self.misfit_dev.clone()
 
     def gradient(self, vel2, stream=None, src_trace: bool = False):
-        """dJ/dvel2 of the last ``misfit`` call (same vel2); with src_trace the
-        adjoint field at the source cell, lambda^{n+1}, is returned as well."""
+        """dJ/dvel2 of the last ``misfit`` call, which must have run on this
+        same vel2 tensor, unmodified since; with src_trace the adjoint field
+        at the source cell, lambda^{n+1}, is returned as well."""
         if not self._have_residual:
             raise RuntimeError("gradient() needs a preceding misfit() on this problem")
         self._check_field(vel2, "vel2")
+        if self._misfit_vel2 != (vel2.data_ptr(), vel2._version):
+            raise RuntimeError("gradient(): vel2 is not the tensor of the last misfit() call on this problem "
+                               "(or it changed since); run misfit() on it first")
         grad = self.torch.empty_like(vel2)
         tr = self.torch.empty(self.n_steps, dtype=self.dtype, device=self.device) if src_trace else None
         fn = getattr(lib(), f"stencil_fwi_backward_{self._sfx}")
@@ -156,6 +170,21 @@ def _sfx_of(t) -> str:
     raise TypeError("fields must be float32 or float64")
 
 
+def _aux_ptr(t, like, name: str, min_numel: int) -> int:
+    """Device pointer of an auxiliary array that the kernels index by a
+    caller-chosen offset: it must sit on the fields' device, be contiguous,
+    of the field dtype, and hold at least min_numel values."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or t.device != like.device:
+        raise ValueError(f"{name} must be a tensor on {like.device}")
+    if t.dtype != like.dtype or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous {like.dtype} tensor")
+    if t.numel() < min_numel:
+        raise ValueError(f"{name} has {t.numel()} values, needs at least {min_numel}")
+    return t.data_ptr()
+
+
 def forward_step(p_cur, p_prev, vel2, p_next, grid: GridDims, lap_out=None, src_index: int = -1, wavelet=None,
                  n: int = 0, variant: str = "tiled", stream=None) -> None:
     """One forward FWI step (stencil_forward_step_*): p_next = 2 p_cur - p_prev
@@ -166,7 +195,15 @@ def forward_step(p_cur, p_prev, vel2, p_next, grid: GridDims, lap_out=None, src_
     ptr = [_dev_ptr(t, grid, dt, nm) for t, nm in ((p_cur, "p_cur"), (p_prev, "p_prev"), (vel2, "vel2"),
                                                        (p_next, "p_next"))]
     lp = _dev_ptr(lap_out, grid, dt, "lap_out") if lap_out is not None else None
-    w = wavelet.data_ptr() if wavelet is not None else None
+    w = None
+    if int(src_index) >= 0:
+        if int(src_index) >= grid.size():
+            raise ValueError(f"src_index {src_index} is outside the grid ({grid.size()} cells)")
+        if wavelet is None or int(n) < 0:
+            raise ValueError("a source (src_index >= 0) needs a wavelet and a sample index n >= 0")
+        w = _aux_ptr(wavelet, p_cur, "wavelet", int(n) + 1)
+    elif wavelet is not None:
+        w = _aux_ptr(wavelet, p_cur, "wavelet", 1)
     g = grid.to_c()
     check(getattr(lib(), f"stencil_forward_step_{sfx}")(*ptr, lp, C.byref(g), int(src_index), w, int(n),
                                                         _variant(variant), _stream(stream)), "forward_step")
@@ -184,11 +221,16 @@ def adjoint_step(psi1, psi2, vel2, psi_out, grid: GridDims, lap=None, grad=None,
     lp = _dev_ptr(lap, grid, dt, "lap") if lap is not None else None
     gp = _dev_ptr(grad, grid, dt, "grad") if grad is not None else None
     op = _dev_ptr(psi_out, grid, dt, "psi_out")
-    if res_plane is not None and (res_plane.dtype != dt or res_plane.numel() != grid.nx * grid.ny
-                                  or not res_plane.is_contiguous()):
-        raise ValueError(f"res_plane must be a contiguous {dt} tensor of nx*ny elements")
-    rp = res_plane.data_ptr() if res_plane is not None else None
-    sv = src_value.data_ptr() if src_value is not None else None
+    rp = None
+    if res_plane is not None:
+        if res_plane.numel() != grid.nx * grid.ny:
+            raise ValueError(f"res_plane must hold nx*ny = {grid.nx * grid.ny} values")
+        rp = _aux_ptr(res_plane, psi1, "res_plane", grid.nx * grid.ny)
+    sv = None
+    if src_value is not None:
+        if not 0 <= int(src_index) < grid.size():
+            raise ValueError(f"src_value needs a src_index inside the grid, got {src_index}")
+        sv = _aux_ptr(src_value, psi1, "src_value", 1)
     g = grid.to_c()
     check(getattr(lib(), f"stencil_adjoint_step_{sfx}")(*ptr, lp, gp, op, rp, int(rec_z), C.byref(g),
                                                         int(src_index), sv, _variant(variant), _stream(stream)),
@@ -206,13 +248,19 @@ def _autograd_function():
             v = vel2.detach().contiguous()
             J = problem.misfit(v, d_obs)
             ctx.problem = problem
-            ctx.save_for_backward(v)
+            ctx.generation = problem.generation
+            ctx.save_for_backward(v, d_obs)
             return J.reshape(())
 
         @staticmethod
         def backward(ctx, grad_out):
-            (v,) = ctx.saved_tensors
-            g = ctx.problem.gradient(v)
+            v, d_obs = ctx.saved_tensors
+            pr = ctx.problem
+            if pr.generation != ctx.generation:
+                # Another misfit/traces call on this problem overwrote the
+                # checkpoints and residual since this forward: rebuild them.
+                pr.misfit(v, d_obs)
+            g = pr.gradient(v)
             return grad_out.to(g.dtype) * g, None, None
 
     return FwiMisfit

@@ -377,6 +377,83 @@ def test_autograd_function(ss, dev):
     assert float(J) == pytest.approx(pr.misfit(md, d_obs).item(), rel=1e-14)
 
 
+def test_autograd_two_misfits_on_one_problem(ss, dev):
+    """Two fwi_misfit calls on one FwiProblem summed before backward(), and a
+    line-search misfit in between: each vel2 still gets its own gradient
+    (the workspace holds only the latest call's checkpoints and residual)."""
+    import torch
+
+    m, w, grid, pr, md = _setup(ss, dev, np.float64)
+    m_true = m.copy()
+    m_true[10:14, 8:12, 10:15] *= 1.1
+    d_obs = pr.traces(torch.from_numpy(m_true).to(dev))
+    m2 = torch.from_numpy(m * 1.02).to(dev)
+    want = []
+    for mm in (md, m2):
+        pr.misfit(mm, d_obs)
+        want.append(pr.gradient(mm).clone())
+    assert not torch.equal(want[0], want[1])
+    a = md.clone().requires_grad_(True)
+    b = m2.clone().requires_grad_(True)
+    J = ss.fwi_misfit(a, pr, d_obs) + ss.fwi_misfit(b, pr, d_obs)
+    pr.misfit(torch.from_numpy(m * 0.97).to(dev), d_obs)  # e.g. a line-search trial
+    J.backward()
+    assert torch.equal(a.grad, want[0]) and torch.equal(b.grad, want[1])
+
+
+def test_gradient_needs_the_misfit_vel2(ss, dev):
+    import torch
+
+    m, w, grid, pr, md = _setup(ss, dev, np.float32)
+    d_obs = pr.traces(md * 1.05)
+    pr.misfit(md, d_obs)
+    with pytest.raises(RuntimeError):
+        pr.gradient(md.clone())  # another tensor
+    md.mul_(1.0)  # same values, but modified in place since the misfit
+    with pytest.raises(RuntimeError):
+        pr.gradient(md)
+    pr.misfit(md, d_obs)
+    assert pr.gradient(md).abs().max() > 0
+
+
+def test_single_steps_reject_bad_auxiliary_arrays(ss, dev):
+    """The wavelet, residual plane and source-value arrays are indexed by the
+    kernels at caller-chosen offsets: a host tensor, another dtype, too few
+    values or an out-of-grid source raise ValueError before any launch."""
+    import torch
+
+    grid = _grid(ss, (20, 20, 24), 10.0, 1e-3, np.float32)
+    f = [torch.zeros((20, 20, 24), device=dev) for _ in range(5)]
+    src = 10 * 24 * 20 + 10 * 24 + 12
+    w = torch.ones(8, device=dev)
+    torch.cuda.synchronize()
+    bad_fwd = [dict(src_index=src, wavelet=w.cpu(), n=0),
+               dict(src_index=src, wavelet=w.double(), n=0),
+               dict(src_index=src, wavelet=w, n=8),
+               dict(src_index=src, wavelet=w, n=-1),
+               dict(src_index=src, wavelet=None, n=0),
+               dict(src_index=grid.size(), wavelet=w, n=0)]
+    for kw in bad_fwd:
+        with pytest.raises(ValueError):
+            ss.forward_step(f[0], f[1], f[2], f[3], grid, **kw)
+    ss.forward_step(f[0], f[1], f[2], f[3], grid, src_index=src, wavelet=w, n=7)  # the last sample is fine
+    plane = torch.zeros(20 * 24, device=dev)
+    sv = torch.zeros(1, device=dev)
+    bad_adj = [dict(res_plane=plane.cpu(), rec_z=6),
+               dict(res_plane=plane.double(), rec_z=6),
+               dict(res_plane=plane[:-1], rec_z=6),
+               dict(src_index=src, src_value=sv.cpu()),
+               dict(src_index=src, src_value=sv.double()),
+               dict(src_index=src, src_value=sv[:0]),
+               dict(src_index=-1, src_value=sv),
+               dict(src_index=grid.size(), src_value=sv)]
+    for kw in bad_adj:
+        with pytest.raises(ValueError):
+            ss.adjoint_step(f[0], f[1], f[2], f[3], grid, **kw)
+    ss.adjoint_step(f[0], f[1], f[2], f[3], grid, res_plane=plane, rec_z=6, src_index=src, src_value=sv)
+    torch.cuda.synchronize()
+
+
 def test_fwi_rejects_bad_configuration(ss, dev):
     import torch
 
