@@ -42,11 +42,15 @@ def parse():
                          "arithmetic (tiled kernel, fused-weight form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the side measurements")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target CPU-baseline sample length per row (two rows: no FTZ/DAZ, FTZ/DAZ)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage / FP64 study")
     ap.add_argument("--precision-n", type=int, nargs="*", default=None,
-                    help="grid sizes of the 16-bit storage study (default: --n)")
+                    help="grid sizes of the 16-bit storage study (default: --n, and 1024 when --n is 512)")
+    ap.add_argument("--precision-steps", type=int, default=1000,
+                    help="steps of the 16-bit storage accuracy runs (the error bar is stated after 1000 steps; "
+                         "independent of --steps)")
     ap.add_argument("--no-fwi", action="store_true", help="skip the adjoint FWI gradient timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-shot timing")
     ap.add_argument("--batch-shots", type=int, default=64, help="shots of the batched modelling")
@@ -293,7 +297,7 @@ def main():
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
     if not args.no_precision:
-        sizes = args.precision_n if args.precision_n else [n]
+        sizes = args.precision_n if args.precision_n else ([n, 1024] if n == 512 else [n])
         result["precision"] = {}
         for pn in sizes:
             if pn == n:
@@ -302,7 +306,8 @@ def main():
                 pshape = (pn, pn, pn)
                 pp0 = torch.from_numpy(ss.host_gaussian(pshape, 0.5, 0.5, 0.5, 3.0)).to(dev)
                 pv2 = torch.from_numpy(ss.host_layered_noise_vel2(pshape, 16, 1500.0, 4000.0, 0.05, 2)).to(dev)
-            result["precision"][str(pn)] = precision_study(ss, pp0, pv2, pn, args.steps, tune=not args.no_tune)
+            result["precision"][str(pn)] = precision_study(ss, pp0, pv2, pn, args.precision_steps,
+                                                           tune=not args.no_tune)
             del pp0, pv2
         torch.cuda.empty_cache()
         p512 = result["precision"].get(str(n))
@@ -310,6 +315,7 @@ def main():
             ok = {k: v for k, v in p512.items() if k in ("bf16", "fp16", "bfp16") and v["rel_l2_vs_fp64"] <= 1e-3}
             best = max(ok, key=lambda k: ok[k]["gpts"]) if ok else None
             result["bf16_store"] = {
+                "accuracy_steps": args.precision_steps, "bar": "rel-L2 <= 1e-3 vs the FP64 GPU run",
                 "bf16_gpts": p512["bf16"]["gpts"], "bf16_rel_l2_vs_fp64": p512["bf16"]["rel_l2_vs_fp64"],
                 "best_16bit_within_1e-3": best,
                 "best_16bit_gpts": ok[best]["gpts"] if best else None,
@@ -352,7 +358,7 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     interior = (n - 8) ** 3
     grid = ss.GridDims.cube(n, np.float32(10.0), np.float32(1e-3))
     red = ss.Reducer(dev)
-    out = {}
+    out = {"steps": steps}
 
     def energy(a, b, v):
         k, p = red.energy(a, b, v, grid)
@@ -438,7 +444,7 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     del ref, v64
     print(f"[bench] precision {n}^3 x {steps}: " +
           ", ".join(f"{k}: {v.get('gpts', '-')} Gpts/s rel {v.get('rel_l2_vs_fp64', 0):.2e}" for k, v in out.items()
-                    if k != "fp64_ref"), file=sys.stderr, flush=True)
+                    if k not in ("fp64_ref", "steps")), file=sys.stderr, flush=True)
     return out
 
 
@@ -571,24 +577,47 @@ def fwi_study(ss, n, steps, dev, tune=True):
     return out
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(p0_h, vel2_h, h, dt, interior, seconds):
     """The reference CPU path (src/kernels/stencil_naive.cu:136-170, OpenMP)
     built from its sources into oracle/_ref/libref_v3.so, timed like
     SolverBase::run in CPU mode on a bounded number of steps of the same
-    workload; the oracle's C restatement if that build is absent."""
+    workload, in two rows: as the reference builds it (denormals kept) and
+    with flush-to-zero / denormals-are-zero set in every OpenMP thread.  The
+    oracle's C restatement (one row) if that build is absent."""
     from oracle.lib import Oracle, RefLib, ref_available
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    omp_env = {k: os.environ[k] for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES", "OMP_SCHEDULE")
+               if k in os.environ}
+    rows = []
     if ref_available("v3"):
         ref = RefLib("v3")
         kind = "reference"
-        one = ref.time_wave_steps(p0_h, vel2_h, h, dt, 0, 1)
-        n_steps = int(max(2, min(60, seconds * 1e3 / max(one, 1e-3))))
-        ms = ref.time_wave_steps(p0_h, vel2_h, h, dt, 1, n_steps)
+        cores = ref.omp_threads()
+        for mode, ftz in (("reference build flags (-O3 -march=x86-64-v3, FMA contraction), denormals kept", False),
+                          ("same build, FTZ/DAZ set in every OpenMP thread", True)):
+            if ref.set_ftz_daz(ftz) != cores:
+                raise RuntimeError("could not set the FTZ/DAZ state in every OpenMP thread")
+            one = ref.time_wave_steps(p0_h, vel2_h, h, dt, 0, 1)
+            n_steps = int(max(2, min(60, seconds * 1e3 / max(one, 1e-3))))
+            ms = ref.time_wave_steps(p0_h, vel2_h, h, dt, 1, n_steps)
+            rows.append({"mode": mode, "ftz_daz": ftz, "ms_per_step": round(ms, 2), "timed_steps": n_steps,
+                         "gpts": round(interior / (ms * 1e-3) / 1e9, 4)})
+        ref.set_ftz_daz(False)
     else:
         import numpy as np
         o = Oracle()
         kind = "port"
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         bufs = [p0_h.copy(), p0_h.copy(), np.zeros_like(p0_h)]
         t = time.perf_counter()
         o.wave_step(bufs[1], bufs[0], vel2_h, bufs[2], h, dt)
@@ -600,13 +629,20 @@ def cpu_baseline(p0_h, vel2_h, h, dt, interior, seconds):
             o.wave_step(bufs[(i + 1) % 3], bufs[i], vel2_h, bufs[(i + 2) % 3], h, dt)
             i = (i + 1) % 3
         ms = (time.perf_counter() - t) * 1e3 / n_steps
-    gpts = interior / (ms * 1e-3) / 1e9
-    print(f"[bench] cpu baseline ({kind}, {cores} threads): {ms:.1f} ms/step, {gpts:.3f} Gpts/s",
-          file=sys.stderr, flush=True)
-    return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": cores, "kind": kind,
-            "ms_per_step": round(ms, 2),
-            "sample": f"{n_steps} timed steps (after 1 warmup) of the same 512^3 workload through the reference "
-                      f"launch_naive CPU path (OpenMP, {cores} threads, no FTZ/DAZ)"}
+        rows.append({"mode": "oracle C restatement (-O2, no contraction), denormals kept", "ftz_daz": False,
+                     "ms_per_step": round(ms, 2), "timed_steps": n_steps,
+                     "gpts": round(interior / (ms * 1e-3) / 1e9, 4)})
+    main = rows[0]
+    for r in rows:
+        print(f"[bench] cpu baseline ({kind}, {cores} threads, {r['mode']}): {r['ms_per_step']:.1f} ms/step, "
+              f"{r['gpts']:.3f} Gpts/s", file=sys.stderr, flush=True)
+    return {"value": main["gpts"], "unit": "Gpts/s", "cores": cores, "kind": kind,
+            "ms_per_step": main["ms_per_step"],
+            "sample": f"{main['timed_steps']} timed steps (after 1 warmup) of the same 512^3 workload through the "
+                      f"reference launch_naive CPU path (OpenMP, {cores} threads, denormals kept as the reference "
+                      f"builds it); FTZ/DAZ row beside it",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(), "omp_env": omp_env,
+            "omp_schedule": "static, collapse(3) (stencil_naive.cu:151)", "rows": rows}
 
 
 if __name__ == "__main__":

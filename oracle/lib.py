@@ -278,6 +278,16 @@ class RefLib:
         L.ref_max_abs_error.restype = _D
         L.ref_time_wave_steps.argtypes = [_P] * 4 + [_SZ] * 3 + [T] * 2 + [C.c_int] * 2
         L.ref_time_wave_steps.restype = _D
+        L.ref_omp_max_threads.restype = C.c_int
+        L.ref_set_ftz_daz.argtypes = [C.c_int]
+        L.ref_set_ftz_daz.restype = C.c_int
+
+    def omp_threads(self) -> int:
+        return int(self.lib.ref_omp_max_threads())
+
+    def set_ftz_daz(self, on: bool) -> int:
+        """FTZ/DAZ in every OpenMP pool thread; returns the threads that took it."""
+        return int(self.lib.ref_set_ftz_daz(int(bool(on))))
 
     def wave_step(self, cur, prev, vel2, nxt, dx, dt):
         self.lib.ref_wave_step(_ptr(cur), _ptr(prev), _ptr(vel2), _ptr(nxt), *_dims(cur), dx, dx, dx, dt)

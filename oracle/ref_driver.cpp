@@ -6,6 +6,9 @@
 // oracle/_ref/.  Nothing here restates the algorithm: every call goes into the
 // reference code, so the outputs are the reference's outputs.  real_t is the
 // reference's (float, or double with -DSTENCIL_FP64).
+#include <omp.h>
+#include <xmmintrin.h>
+
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -21,6 +24,25 @@ using namespace stencil;
 extern "C" {
 
 int ref_sizeof_real() { return static_cast<int>(sizeof(real_t)); }
+
+// Threads the reference's OpenMP loop (stencil_naive.cu:151) runs with.
+int ref_omp_max_threads() { return omp_get_max_threads(); }
+
+// Flush-to-zero and denormals-are-zero (MXCSR bits 15 and 6) on or off in
+// every thread of the OpenMP pool, which later parallel regions reuse; the
+// reference's CPU build leaves both off (its flags: CMakeLists.txt:109).
+// Returns how many pool threads report the requested state.
+int ref_set_ftz_daz(int on) {
+    int ok = 0;
+#pragma omp parallel reduction(+ : ok)
+    {
+        const unsigned bits = 0x8040u;
+        const unsigned csr = _mm_getcsr();
+        _mm_setcsr(on ? (csr | bits) : (csr & ~bits));
+        ok += ((_mm_getcsr() & bits) == (on ? bits : 0u)) ? 1 : 0;
+    }
+    return ok;
+}
 
 // One launch_naive call (include/stencil/kernels.cuh:42) on host buffers.
 void ref_wave_step(const real_t* cur, const real_t* prev, const real_t* vel2, real_t* next,
