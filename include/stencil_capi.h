@@ -13,7 +13,8 @@
  * object allocates its small pointer tables at create, and the FWI backward
  * pass runs on a second stream of its own (created once per device) joined
  * to the caller's by events it creates per call; device memory beyond the
- * inputs and outputs is always the caller's (workspaces).
+ * inputs and outputs is the caller's (workspaces), except for the static
+ * per-device partial-sum array of stencil_energy_f32_noalloc.
  */
 #ifndef STENCIL_CAPI_H
 #define STENCIL_CAPI_H
@@ -223,6 +224,19 @@ int stencil_energy_f32(const float* p_n, const float* p_np1, const float* vel2,
                        const stencil_grid* grid, double* out, double* work, void* hip_stream);
 int stencil_energy_f64(const double* p_n, const double* p_np1, const double* vel2,
                        const stencil_grid* grid, double* out, double* work, void* hip_stream);
+/* No counterpart in the reference.  stencil_energy_f32 for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * or time-step loops, harnesses that forbid hipMalloc): the same kernels and
+ * results, with the partial sums in a static per-device array of the library
+ * (24 KiB: 3 x 1024 doubles, the same for every grid) instead of `work`.  The
+ * array's address is resolved on the first call on a device, with that device
+ * current (call with the stream's device current).  Allocates nothing;
+ * asynchronous on hip_stream like the other reductions.  Calls that overlap on
+ * one device (other streams, other host threads) share the array and race:
+ * order them.  Grids the reductions cannot take (an axis of <= 8 or >= 2^20
+ * cells) return STENCIL_ERR_INVALID. */
+int stencil_energy_f32_noalloc(const float* p_n, const float* p_np1, const float* vel2,
+                               const stencil_grid* grid, double* out, void* hip_stream);
 /* out[0] = sum (a-b)^2, out[1] = sum b^2 (relative L2 = sqrt(out0/out1)). */
 int stencil_diff_norms_f32_f64(const float* a, const double* b, const stencil_grid* grid,
                                double* out, double* work, void* hip_stream);

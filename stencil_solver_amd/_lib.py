@@ -103,6 +103,7 @@ SIGNATURES = {
                                                                             C.POINTER(stencil_tile_shape), C.c_char_p, _P]),
     "stencil_reduce_workspace_bytes": (C.c_uint64, []),
     "stencil_energy_f32": (_I, [_P, _P, _P, _G, _P, _P, _P]),
+    "stencil_energy_f32_noalloc": (_I, [_P, _P, _P, _G, _P, _P]),
     "stencil_energy_f64": (_I, [_P, _P, _P, _G, _P, _P, _P]),
     "stencil_diff_norms_f32_f64": (_I, [_P, _P, _G, _P, _P, _P]),
     "stencil_dot_f64": (_I, [_P, _P, _G, _P, _P, _P]),

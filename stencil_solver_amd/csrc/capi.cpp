@@ -15,14 +15,16 @@
 using namespace stencil::hipk;
 
 namespace {
-
 thread_local int g_last_hip = 0;
+}  // namespace
 
-int hip_status(hipError_t e) {
+int stencil::hipk::hip_status(hipError_t e) {
     if (e == hipSuccess) return STENCIL_OK;
     g_last_hip = static_cast<int>(e);
     return STENCIL_ERR_HIP;
 }
+
+namespace {
 
 // Dims must be non-zero and the field must fit size_t indexing; dims of at
 // most 2R on an axis leave no interior and make the launch a no-op (the

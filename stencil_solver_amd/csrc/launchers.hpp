@@ -133,4 +133,7 @@ hipError_t heat_naive_f64(const double* t, double* tn, const Dims& d, const Scal
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,
                        float coef, bool exact, ZqShape shape, hipStream_t st);
 
+// STENCIL_OK, or STENCIL_ERR_HIP with e kept for stencil_last_hip_error().
+int hip_status(hipError_t e);
+
 }  // namespace stencil::hipk

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "device_common.hpp"
 #include "launchers.hpp"
@@ -216,8 +217,36 @@ hipError_t decode16(int fmt, Field16 src, float* dst, size_t n, float scale, hip
 
 using namespace stencil::hipk;
 
+// Partial sums of stencil_energy_f32_noalloc: kMaxSums x kRedBlocks doubles
+// (24 KiB), one copy per device in the library's code object, held until the
+// process ends.  In host code this name is a stand-in, not a device address:
+// kernels get the per-device address that hipGetSymbolAddress returns.
+static __device__ __align__(256) double scratch[kMaxSums * kRedBlocks];
+
 namespace {
 int status_of(hipError_t e) { return e == hipSuccess ? STENCIL_OK : STENCIL_ERR_HIP; }
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxScratchDevices] = {};
+
+// The current device's copy of `scratch`, resolved on the first call on that
+// device (with it current) and kept; no allocation is involved.
+hipError_t scratch_of_current_device(double** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
 }  // namespace
 
 extern "C" {
@@ -229,6 +258,19 @@ int stencil_energy_f32(const float* p_n, const float* p_np1, const float* vel2, 
     if (!p_n || !p_np1 || !vel2 || !out2 || !work || !red_grid_ok(grid)) return STENCIL_ERR_INVALID;
     return status_of(reduce<0, float, float>(red_args(grid, p_n, p_np1, vel2), work, out2,
                                              static_cast<hipStream_t>(stream)));
+}
+
+int stencil_energy_f32_noalloc(const float* p_n, const float* p_np1, const float* vel2,
+                               const stencil_grid* grid, double* out2, void* stream) {
+    if (!p_n || !p_np1 || !vel2 || !out2 || !red_grid_ok(grid)) return STENCIL_ERR_INVALID;
+    static_assert(sizeof(scratch) == sizeof(double) * kMaxSums * kRedBlocks, "partials fill the scratch");
+    double* part = nullptr;
+    const hipError_t e = scratch_of_current_device(&part);
+    if (e != hipSuccess) return hip_status(e);
+    // Every workgroup of reduce_partial overwrites its own partials before
+    // reduce_final reads them (stream order), so nothing needs clearing.
+    return hip_status(reduce<0, float, float>(red_args(grid, p_n, p_np1, vel2), part, out2,
+                                              static_cast<hipStream_t>(stream)));
 }
 
 int stencil_energy_f64(const double* p_n, const double* p_np1, const double* vel2,
