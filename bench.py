@@ -491,7 +491,9 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
     sizes_hist = {int(k): int(v) for k, v in zip(*np.unique(sizes, return_counts=True))}
     out = {"shots": n_shots, "steps": steps, "sizes": sizes_hist, "interior_points_per_step": pts,
            "run_ms": round(ms, 1), "ms_per_step": round(ms / steps, 4), "gpts": round(gpts, 1),
-           "achieved_gbs": round(gbs, 1), "graph_build_s": round(t_build, 2), "inputs_s": round(t_inputs, 2),
+           "achieved_gbs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+           "graph_build_s": round(t_build, 2), "inputs_s": round(t_inputs, 2),
+           "mode": "ragged: one launch per step over all shots" if b.ragged else "grouped: one launch per grid per step",
            "groups": b.groups()}
     print(f"[bench] batch {n_shots} shots x {steps}: {ms:.0f} ms, {gpts:.1f} Gpts/s, {gbs:.0f} GB/s "
           f"(graph build {t_build:.1f}s)", file=sys.stderr, flush=True)

@@ -350,14 +350,20 @@ int stencil_adjoint_step_f64(const double* psi1, const double* psi2, const doubl
 /* ---- Batched multi-shot forward modelling (no reference counterpart).
  * Each shot is the FWI forward scheme on its own grid: three caller-owned
  * time-level buffers, vel2, and an optional point source (wavelet of n_steps
- * device samples; NULL = none).  Shots are grouped by grid; each group's
- * launch shape comes from the wave autotuner (run at create time when tune is
- * non-zero, else its cache or the default); the whole run -- zeroing all
- * levels, then n_steps steps of every group, one launch per group per step --
- * is captured into one hipGraph at create time.  run() is one graph launch;
- * afterwards p^{n_steps} of every shot is in ring[stencil_batch_newest()].
- * Every run starts again from zero fields.  Needs nx % 4 == 0 and 16-byte
- * aligned buffers. ---- */
+ * device samples; NULL = none).  Every time step of every shot, whatever its
+ * grid, is ONE launch of the tiled kernel over a table of work items (shot,
+ * tile, z-chunk) built at create time: one tile shape for the batch, z-chunks
+ * of about a target length, the items cut into eight equal-cost runs, one
+ * per XCD.  With tune non-zero the batch autotuner times that launch on the
+ * batch's buffers for every built tile shape, a few chunk lengths and with /
+ * without the XCD runs, and keeps the fastest (else a default plan).  The
+ * whole run -- zeroing all levels, then n_steps launches -- is captured into
+ * one hipGraph at create time.  run() is one graph launch; afterwards
+ * p^{n_steps} of every shot is in ring[stencil_batch_newest()].  Every run
+ * starts again from zero fields.  Needs nx % 4 == 0 and 16-byte aligned
+ * buffers.  With STENCIL_BATCH_MODE=grouped in the environment at create
+ * time, shots of equal grids instead form groups with one launch per group
+ * per step, each group's shape from the wave autotuner (same results). ---- */
 typedef struct stencil_shot {
     stencil_grid grid;
     float* ring[3];
@@ -371,10 +377,21 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
 int stencil_batch_run(stencil_batch* batch, void* hip_stream);
 /* Ring index holding the newest level (same for every shot). */
 int stencil_batch_newest(const stencil_batch* batch, int shot);
-/* Groups of equal grids, and per group its grid, launch shape and shot count. */
+/* Groups of equal grids, and per group its grid, launch shape and shot count
+ * (ragged: the batch's tile shape, this grid's z-chunk count, xcd_remap = the
+ * per-XCD runs). */
 int stencil_batch_groups(const stencil_batch* batch);
 int stencil_batch_group(const stencil_batch* batch, int group, stencil_grid* grid,
                         stencil_tile_shape* shape, int* n_shots);
+/* Re-plans a (ragged) batch with an explicit plan and recaptures its graph:
+ * the tile shape (tile_x4, tile_y), z-chunks of about chunk_planes planes,
+ * xcd_remap 1 for the per-XCD runs (0: one list, longest items first);
+ * n_chunks and nontemporal are ignored.  STENCIL_ERR_UNSUPPORTED for a
+ * grouped batch.  Synchronises. */
+int stencil_batch_set_plan(stencil_batch* batch, const stencil_tile_shape* shape, int chunk_planes,
+                           void* hip_stream);
+/* 1 for a ragged batch (one launch per step), 0 for a grouped one. */
+int stencil_batch_is_ragged(const stencil_batch* batch);
 void stencil_batch_destroy(stencil_batch* batch);
 
 /* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)

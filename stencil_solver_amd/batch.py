@@ -1,7 +1,8 @@
 """Batched multi-shot forward modelling (include/stencil_capi.h,
-stencil_batch_*): many shots on their own grids stepped together, one
-hipGraph launch per run.  No reference counterpart (the reference runs one
-grid per process).
+stencil_batch_*): many shots on their own grids stepped together -- one
+launch of the tiled kernel per time step over a work-item table spanning
+every shot, the whole run one hipGraph launch.  No reference counterpart (the
+reference runs one grid per process).
 
 Each shot is the FWI forward scheme (stencil_solver_amd.fwi) from zero fields:
 p^{n+1} = 2 p^n - p^{n-1} + dt^2 vel2 L p^n + wavelet[n] at the source cell.
@@ -61,6 +62,22 @@ class ShotBatch:
     def run(self, stream=None) -> None:
         """All shots, n_steps each, from zero fields: one graph launch."""
         check(lib().stencil_batch_run(self._h, _stream(stream)), "batch_run")
+
+    @property
+    def ragged(self) -> bool:
+        """True when every step of every shot is one launch (the default);
+        False for the grouped mode (STENCIL_BATCH_MODE=grouped at creation)."""
+        return lib().stencil_batch_is_ragged(self._h) == 1
+
+    def set_plan(self, tile_x: int, tile_y: int, chunk_planes: int, xcd_runs: bool = True, stream=None) -> None:
+        """Re-plan the ragged launch: tiles of tile_x x tile_y cells (a built
+        shape), z-chunks of about chunk_planes planes, the per-XCD runs on or
+        off; recaptures the graph (synchronises)."""
+        if tile_x % 4:
+            raise ValueError("tile_x must be a multiple of 4 cells")
+        shp = stencil_tile_shape(tile_x // 4, int(tile_y), 0, int(bool(xcd_runs)), 1)
+        check(lib().stencil_batch_set_plan(self._h, C.byref(shp), int(chunk_planes), _stream(stream)),
+              "batch_set_plan")
 
     def newest(self, i: int):
         """p^{n_steps} of shot i (valid after run())."""

@@ -1,19 +1,38 @@
 // Batched multi-shot forward modelling (no reference counterpart: the
 // reference runs one grid per process).
 //
-// Shots are grouped by grid (dims, spacing, dt).  For each group the launch
-// shape is the wave autotuner's (run on the group's first shot when asked, or
-// taken from its cache / the default), and one launch of the tiled forward
-// kernel steps every shot of the group (workgroup row = shot, pointers from a
-// device table).  The whole run -- zeroing the three time levels of every
-// shot, then n_steps steps of every group -- is captured once into a hipGraph
-// whose group chains are independent branches (forked capture streams), so a
-// run is one graph launch and small groups overlap on the GPU.
+// Ragged mode (the default): every time step of every shot, whatever its
+// grid, is ONE launch of the tiled forward kernel over a table of work items
+// (shot, tile, z-chunk), so the small grids fill the CUs together with the
+// large ones instead of running as separate, under-filled launches.  The
+// table is built on the host:
+//   * each shot is tiled with the batch's tile shape and cut into z-chunks of
+//     about `chunk_target` planes;
+//   * the items (shots in order of decreasing size, each shot's tiles in row
+//     order) are cut into eight runs of equal estimated cost, one per XCD, so
+//     every XCD has the same work and a shot's neighbouring tiles share an L2;
+//   * each run is ordered longest item first (the short ones fill the tail),
+//     and the runs are interleaved so that workgroup b, which the dispatcher
+//     sends to XCD b % 8, is the next item of that XCD's run (a shorter run is
+//     padded with empty items).
+// The batch autotuner times this launch itself on the batch's buffers for
+// every built tile shape and a few chunk lengths, with and without the XCD
+// runs, and keeps the fastest.  The whole run -- zeroing the three time levels
+// of every shot, then n_steps launches -- is captured once into a hipGraph;
+// a run is one graph launch.
+//
+// Grouped mode (STENCIL_BATCH_MODE=grouped, kept for comparison): shots of
+// equal grids form a group; each group has its own tuned shape and one launch
+// per step (workgroup row = shot), the group chains being independent graph
+// branches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <new>
+#include <numeric>
 #include <tuple>
 #include <vector>
 
@@ -26,17 +45,34 @@ hipError_t fill_f32(float*, size_t, float, hipStream_t);
 
 using namespace stencil::hipk;
 
+namespace {
+
+// A ragged launch plan: tile shape, z-chunk length target, XCD runs on/off.
+struct RaggedPlan {
+    int txv = 16, ty = 16;
+    int chunk_target = 90;
+    int xcd_runs = 1;
+};
+
+}  // namespace
+
 struct stencil_batch {
     struct Group {
         Dims d;
         Scalars<float> s;
         ZqShape shape;
         std::vector<int> shots;
-        BatchShot* table = nullptr;  // device
+        BatchShot* table = nullptr;  // device (grouped mode)
+        int n_chunks = 0;            // ragged mode: z-chunks per column of this grid
     };
     std::vector<Group> groups;
     std::vector<stencil_shot> shots;
     int n_steps = 0;
+    bool ragged = true;
+    RaggedPlan plan;
+    RaggedShot* d_shots = nullptr;  // device, one per shot
+    RaggedItem* d_items = nullptr;  // device, n_items
+    int n_items = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -60,13 +96,177 @@ bool shot_ok(const stencil_shot& s) {
     return zq_supported(d, ptrs, 4);
 }
 
+bool batch_mode_grouped() {
+    const char* e = std::getenv("STENCIL_BATCH_MODE");
+    return e && std::strcmp(e, "grouped") == 0;
+}
+
 void release(stencil_batch* b) {
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto& g : b->groups)
         if (g.table) (void)hipFree(g.table);
+    if (b->d_shots) (void)hipFree(b->d_shots);
+    if (b->d_items) (void)hipFree(b->d_items);
     delete b;
+}
+
+// ---- ragged plan -------------------------------------------------------
+
+// Per-shot tiling for this plan, and the work items in dispatch order.
+void build_plan(const std::vector<stencil_shot>& shots, const RaggedPlan& p, std::vector<RaggedShot>& rs,
+                std::vector<RaggedItem>& out, std::vector<int>* chunks_per_shot = nullptr) {
+    const int n = int(shots.size());
+    rs.assign(n, RaggedShot{});
+    struct Item {
+        RaggedItem it;
+        double cost;
+    };
+    std::vector<std::vector<Item>> per_shot(n);
+    if (chunks_per_shot) chunks_per_shot->assign(n, 0);
+    const double threads = double(p.txv) * double(p.ty);
+    for (int i = 0; i < n; ++i) {
+        const stencil_shot& sh = shots[i];
+        RaggedShot& r = rs[i];
+        for (int k = 0; k < 3; ++k) r.ring[k] = sh.ring[k];
+        r.vel2 = sh.vel2;
+        r.wavelet = sh.wavelet;
+        r.src_z = int(sh.src_z);
+        r.src_y = int(sh.src_y);
+        r.src_x = int(sh.src_x);
+        r.nx = int(sh.grid.nx);
+        r.ny = int(sh.grid.ny);
+        r.nz = int(sh.grid.nz);
+        r.tiles_x = int((sh.grid.nx + 4 * p.txv - 1) / (4 * p.txv));
+        r.tiles_y = int((sh.grid.ny - 8 + p.ty - 1) / p.ty);
+        const int64_t inz = int64_t(sh.grid.nz) - 8;
+        int64_t want = (inz + p.chunk_target / 2) / std::max(1, p.chunk_target);
+        if (want < 1) want = 1;
+        r.chunk = zq_chunk_planes(inz, int(want));
+        // a workgroup's buffer resources span its chunk plus 21 planes: keep
+        // them below 2^32 bytes (zq_supported guarantees a 9-plane chunk fits)
+        const int64_t max_chunk = (int64_t(0xFFFFFFFF) / (int64_t(sh.grid.nx) * int64_t(sh.grid.ny) * 4) - 21) / 9 * 9;
+        if (r.chunk > max_chunk) r.chunk = int(max_chunk);
+        r.s = make_scalars<float>(float(sh.grid.dx), float(sh.grid.dy), float(sh.grid.dz), float(sh.grid.dt));
+        const int n_ch = int((inz + r.chunk - 1) / r.chunk);
+        if (chunks_per_shot) (*chunks_per_shot)[i] = n_ch;
+        for (int c = 0; c < n_ch; ++c) {
+            const int64_t planes = std::min<int64_t>(r.chunk, inz - int64_t(c) * r.chunk);
+            // steps run in whole groups of 9, after an 8-plane queue fill
+            const double cost = threads * (double((planes + 8) / 9 * 9) + 4.0);
+            for (int ty = 0; ty < r.tiles_y; ++ty)
+                for (int tx = 0; tx < r.tiles_x; ++tx) per_shot[i].push_back({{i, tx, ty, c}, cost});
+        }
+    }
+    // shots in order of decreasing size (ties: input order)
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const auto sa = shots[a].grid.nx * shots[a].grid.ny * shots[a].grid.nz;
+        const auto sb = shots[b].grid.nx * shots[b].grid.ny * shots[b].grid.nz;
+        return sa > sb;
+    });
+    std::vector<Item> seq;
+    for (int i : order) seq.insert(seq.end(), per_shot[i].begin(), per_shot[i].end());
+    auto by_cost = [](const Item& a, const Item& b) { return a.cost > b.cost; };
+    out.clear();
+    if (!p.xcd_runs) {
+        std::stable_sort(seq.begin(), seq.end(), by_cost);
+        for (const Item& x : seq) out.push_back(x.it);
+        return;
+    }
+    constexpr int kXcds = 8;
+    double total = 0.0;
+    for (const Item& x : seq) total += x.cost;
+    std::vector<std::vector<Item>> runs(kXcds);
+    double before = 0.0;
+    for (const Item& x : seq) {
+        // the run whose cost interval holds this item's midpoint
+        int k = int((before + 0.5 * x.cost) / total * kXcds);
+        k = std::min(std::max(k, 0), kXcds - 1);
+        runs[k].push_back(x);
+        before += x.cost;
+    }
+    size_t longest = 0;
+    for (auto& r : runs) {
+        std::stable_sort(r.begin(), r.end(), by_cost);
+        longest = std::max(longest, r.size());
+    }
+    for (size_t j = 0; j < longest; ++j)
+        for (int k = 0; k < kXcds; ++k) out.push_back(j < runs[k].size() ? runs[k][j].it : RaggedItem{-1, 0, 0, 0});
+}
+
+// Uploads the plan's shot table and items (replacing the batch's).
+hipError_t upload_plan(stencil_batch* b, const RaggedPlan& p, hipStream_t st) {
+    std::vector<RaggedShot> rs;
+    std::vector<RaggedItem> items;
+    std::vector<int> chunks;
+    build_plan(b->shots, p, rs, items, &chunks);
+    if (b->d_shots) (void)hipFree(b->d_shots);
+    if (b->d_items) (void)hipFree(b->d_items);
+    b->d_shots = nullptr;
+    b->d_items = nullptr;
+    b->n_items = 0;
+    hipError_t e = hipMalloc(&b->d_shots, rs.size() * sizeof(RaggedShot));
+    if (e == hipSuccess) e = hipMalloc(&b->d_items, items.size() * sizeof(RaggedItem));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_shots, rs.data(), rs.size() * sizeof(RaggedShot), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(RaggedItem), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    b->n_items = int(items.size());
+    b->plan = p;
+    for (auto& g : b->groups) g.n_chunks = chunks[g.shots.front()];
+    return hipSuccess;
+}
+
+hipError_t zero_rings(const stencil_batch* b, hipStream_t st) {
+    for (const stencil_shot& s : b->shots) {
+        const size_t cells = size_t(s.grid.nx) * size_t(s.grid.ny) * size_t(s.grid.nz);
+        for (float* r : s.ring) {
+            const hipError_t e = fill_f32(r, cells, 0.f, st);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// The batch autotuner: every built tile shape x chunk target x XCD runs on /
+// off, each timed over `reps` launches of the ragged step on the batch's own
+// buffers (zeroed first; the run starts from zero again anyway).
+hipError_t tune_plan(stencil_batch* b, int reps, hipStream_t st) {
+    static const int shapes[][2] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}, {64, 12}, {64, 16}};
+    hipError_t e = zero_rings(b, st);
+    if (e != hipSuccess) return e;
+    RaggedPlan best;
+    float best_ms = -1.f;
+    for (const auto& sh : shapes)
+        for (int target : {54, 90, 144, 4096})
+            for (int runs : {1, 0}) {
+                RaggedPlan p;
+                p.txv = sh[0];
+                p.ty = sh[1];
+                p.chunk_target = target;
+                p.xcd_runs = runs;
+                e = upload_plan(b, p, st);
+                if (e != hipSuccess) return e;
+                int step = 0;
+                const float ms = zq_time_launches(
+                    [&] {
+                        const int n = step++;
+                        return ragged_zq_fwi_f32(b->d_shots, b->d_items, b->n_items, n % 3, n % b->n_steps, p.txv,
+                                                 p.ty, true, st);
+                    },
+                    reps, st);
+                if (ms < 0.f) return hipErrorLaunchFailure;
+                if (best_ms < 0.f || ms < best_ms) {
+                    best_ms = ms;
+                    best = p;
+                }
+            }
+    return upload_plan(b, best, st);
 }
 
 #define B_TRY(x)                        \
@@ -79,9 +279,11 @@ void release(stencil_batch* b) {
 
 int build_graph(stencil_batch* b) {
     // Captured on a stream of its own (the caller's may be the legacy default
-    // stream, which cannot capture); one forked stream per group joins it.
-    std::vector<hipStream_t> streams(b->groups.size(), nullptr);
-    std::vector<hipEvent_t> done(b->groups.size(), nullptr);
+    // stream, which cannot capture); in grouped mode one forked stream per
+    // group joins it.
+    const size_t n_branches = b->ragged ? 0 : b->groups.size();
+    std::vector<hipStream_t> streams(n_branches, nullptr);
+    std::vector<hipEvent_t> done(n_branches, nullptr);
     hipEvent_t fork = nullptr;
     hipStream_t st = nullptr;
     bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
@@ -93,20 +295,27 @@ int build_graph(stencil_batch* b) {
     if (ok && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) ok = false;
     if (ok && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) ok = false;
     if (ok) {
-        B_TRY(hipEventRecord(fork, st));
-        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-            auto& g = b->groups[gi];
-            hipStream_t gs = streams[gi];
-            B_TRY(hipStreamWaitEvent(gs, fork, 0));
-            const size_t cells = size_t(g.d.nx) * size_t(g.d.ny) * size_t(g.d.nz);
-            // p^{-1} = p^0 = 0 (and the third level, so halos stay zero): a fill
-            // kernel, not a memset node
-            for (int si : g.shots)
-                for (float* r : b->shots[si].ring) B_TRY(fill_f32(r, cells, 0.f, gs));
+        if (b->ragged) {
+            // p^{-1} = p^0 = 0 (and the third level, so halos stay zero): fill
+            // kernels, not memset nodes
+            B_TRY(zero_rings(b, st));
             for (int n = 0; n < b->n_steps; ++n)
-                B_TRY(batch_zq_fwi_f32(g.table, int(g.shots.size()), n % 3, n, g.d, g.s, g.shape, gs));
-            B_TRY(hipEventRecord(done[gi], gs));
-            B_TRY(hipStreamWaitEvent(st, done[gi], 0));
+                B_TRY(ragged_zq_fwi_f32(b->d_shots, b->d_items, b->n_items, n % 3, n, b->plan.txv, b->plan.ty, true,
+                                        st));
+        } else {
+            B_TRY(hipEventRecord(fork, st));
+            for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+                auto& g = b->groups[gi];
+                hipStream_t gs = streams[gi];
+                B_TRY(hipStreamWaitEvent(gs, fork, 0));
+                const size_t cells = size_t(g.d.nx) * size_t(g.d.ny) * size_t(g.d.nz);
+                for (int si : g.shots)
+                    for (float* r : b->shots[si].ring) B_TRY(fill_f32(r, cells, 0.f, gs));
+                for (int n = 0; n < b->n_steps; ++n)
+                    B_TRY(batch_zq_fwi_f32(g.table, int(g.shots.size()), n % 3, n, g.d, g.s, g.shape, gs));
+                B_TRY(hipEventRecord(done[gi], gs));
+                B_TRY(hipStreamWaitEvent(st, done[gi], 0));
+            }
         }
     capture_end:
         hipGraph_t graph = nullptr;
@@ -128,6 +337,39 @@ int build_graph(stencil_batch* b) {
     return ok ? STENCIL_OK : STENCIL_ERR_HIP;
 }
 
+// Grouped mode: each group's shape and device table.
+int setup_groups(stencil_batch* b, int tune, void* stream) {
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    for (auto& g : b->groups) {
+        const stencil_shot& s0 = b->shots[g.shots.front()];
+        if (tune) {
+            // times candidate shapes on the first shot's buffers (overwrites ring[2])
+            stencil_tile_shape best;
+            const int rc = stencil_tune_tiled_f32(s0.ring[1], s0.ring[0], s0.vel2, s0.ring[2], &s0.grid, 3, &best,
+                                                  nullptr, stream);
+            if (rc != STENCIL_OK) return rc;
+        }
+        if (!zq_cached_shape(g.d, 0, &g.shape)) g.shape = zq_resolve(g.d, ZqShape{}, 0);
+        std::vector<BatchShot> host(g.shots.size());
+        for (size_t i = 0; i < g.shots.size(); ++i) {
+            const stencil_shot& s = b->shots[g.shots[i]];
+            for (int r = 0; r < 3; ++r) host[i].ring[r] = s.ring[r];
+            host[i].vel2 = s.vel2;
+            host[i].wavelet = s.wavelet;
+            host[i].src_z = int(s.src_z);
+            host[i].src_y = int(s.src_y);
+            host[i].src_x = int(s.src_x);
+            host[i].pad = 0;
+        }
+        if (hipMalloc(&g.table, host.size() * sizeof(BatchShot)) != hipSuccess ||
+            hipMemcpyAsync(g.table, host.data(), host.size() * sizeof(BatchShot), hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return STENCIL_ERR_HIP;
+    }
+    return STENCIL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -144,7 +386,8 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
     if (!b) return STENCIL_ERR_HIP;
     b->shots.assign(shots, shots + n_shots);
     b->n_steps = n_steps;
-    // group by grid
+    b->ragged = !batch_mode_grouped();
+    // group by grid (the grouped mode's launches; reported in either mode)
     using Key = std::tuple<uint64_t, uint64_t, uint64_t, double, double, double, double>;
     std::map<Key, int> index;
     for (int i = 0; i < n_shots; ++i) {
@@ -162,39 +405,14 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
         }
         b->groups[it->second].shots.push_back(i);
     }
-    for (auto& g : b->groups) {
-        const stencil_shot& s0 = b->shots[g.shots.front()];
-        if (tune) {
-            // times candidate shapes on the first shot's buffers (overwrites ring[2])
-            stencil_tile_shape best;
-            const int rc = stencil_tune_tiled_f32(s0.ring[1], s0.ring[0], s0.vel2, s0.ring[2], &s0.grid, 3, &best,
-                                                  nullptr, stream);
-            if (rc != STENCIL_OK) {
-                release(b);
-                return rc;
-            }
-        }
-        if (!zq_cached_shape(g.d, 0, &g.shape)) g.shape = zq_resolve(g.d, ZqShape{}, 0);
-        std::vector<BatchShot> host(g.shots.size());
-        for (size_t i = 0; i < g.shots.size(); ++i) {
-            const stencil_shot& s = b->shots[g.shots[i]];
-            for (int r = 0; r < 3; ++r) host[i].ring[r] = s.ring[r];
-            host[i].vel2 = s.vel2;
-            host[i].wavelet = s.wavelet;
-            host[i].src_z = int(s.src_z);
-            host[i].src_y = int(s.src_y);
-            host[i].src_x = int(s.src_x);
-            host[i].pad = 0;
-        }
-        if (hipMalloc(&g.table, host.size() * sizeof(BatchShot)) != hipSuccess ||
-            hipMemcpyAsync(g.table, host.data(), host.size() * sizeof(BatchShot), hipMemcpyHostToDevice, st) !=
-                hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            release(b);
-            return STENCIL_ERR_HIP;
-        }
+    int rc = STENCIL_OK;
+    if (b->ragged) {
+        const hipError_t e = tune ? tune_plan(b, 4, st) : upload_plan(b, RaggedPlan{}, st);
+        if (e != hipSuccess) rc = STENCIL_ERR_HIP;
+    } else {
+        rc = setup_groups(b, tune, stream);
     }
-    const int rc = build_graph(b);
+    if (rc == STENCIL_OK) rc = build_graph(b);
     if (rc != STENCIL_OK) {
         release(b);
         return rc;
@@ -221,16 +439,44 @@ int stencil_batch_group(const stencil_batch* b, int group, stencil_grid* grid, s
     const auto& g = b->groups[group];
     if (grid) *grid = b->shots[g.shots.front()].grid;
     if (shape) {
-        const ZqShape r = zq_resolve(g.d, g.shape, 0);
-        shape->tile_x4 = r.txv;
-        shape->tile_y = r.ty;
-        shape->n_chunks = r.n_chunks;
-        shape->xcd_remap = r.xcd_remap;
+        if (b->ragged) {
+            // one tile shape for the whole batch; this grid's z-chunk count;
+            // xcd_remap reports the per-XCD runs
+            shape->tile_x4 = b->plan.txv;
+            shape->tile_y = b->plan.ty;
+            shape->n_chunks = g.n_chunks;
+            shape->xcd_remap = b->plan.xcd_runs;
+        } else {
+            const ZqShape r = zq_resolve(g.d, g.shape, 0);
+            shape->tile_x4 = r.txv;
+            shape->tile_y = r.ty;
+            shape->n_chunks = r.n_chunks;
+            shape->xcd_remap = r.xcd_remap;
+        }
         shape->nontemporal = 1;
     }
     if (n_shots) *n_shots = int(g.shots.size());
     return STENCIL_OK;
 }
+
+int stencil_batch_set_plan(stencil_batch* b, const stencil_tile_shape* shape, int chunk_planes, void* stream) {
+    if (!b || !shape || chunk_planes < 1 || !zq_shape_valid(shape->tile_x4, shape->tile_y)) return STENCIL_ERR_INVALID;
+    if (!b->ragged) return STENCIL_ERR_UNSUPPORTED;
+    if (hipDeviceSynchronize() != hipSuccess) return STENCIL_ERR_HIP;  // no run of the old graph in flight
+    RaggedPlan p;
+    p.txv = shape->tile_x4;
+    p.ty = shape->tile_y;
+    p.chunk_target = chunk_planes;
+    p.xcd_runs = shape->xcd_remap != 0 ? 1 : 0;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (upload_plan(b, p, static_cast<hipStream_t>(stream)) != hipSuccess) return STENCIL_ERR_HIP;
+    return build_graph(b);
+}
+
+int stencil_batch_is_ragged(const stencil_batch* b) { return b ? (b->ragged ? 1 : 0) : STENCIL_ERR_INVALID; }
 
 void stencil_batch_destroy(stencil_batch* b) { release(b); }
 

@@ -116,6 +116,30 @@ struct BatchShot {
     const float* wavelet;
     int src_z, src_y, src_x, pad;
 };
+// One shot of a ragged batched launch (shots of any dims in one launch): its
+// buffers and source as in BatchShot, plus its own dims, tiling (tiles per
+// plane row / column, z-chunk length) and scalars.
+struct RaggedShot {
+    float* ring[3];
+    const float* vel2;
+    const float* wavelet;
+    int src_z, src_y, src_x;
+    int nx, ny, nz;
+    int tiles_x, tiles_y, chunk;
+    Scalars<float> s;
+};
+// One workgroup of a ragged launch: tile (tile_x, tile_y) of z-chunk chunk_id
+// of shot `shot`; shot < 0 = no work (padding of a shorter per-XCD list).
+struct RaggedItem {
+    int shot, tile_x, tile_y, chunk_id;
+};
+// The forward FWI step of every shot of a ragged batch in one launch of
+// n_items workgroups of txv * ty threads: workgroup b steps items[b], shot i
+// stepping ring[rot] / ring[rot+1] -> ring[rot+2] (mod 3) and adding
+// wavelet[step].  Every shot's tiling must be the one its items were built
+// with for this (txv, ty).
+hipError_t ragged_zq_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int rot, int step,
+                             int txv, int ty, bool nt, hipStream_t st);
 // The forward FWI step of n_shots shots of equal dims in one launch: shot i
 // steps ring[rot] / ring[rot+1] -> ring[rot+2] (mod 3), adding wavelet[step].
 hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int step, const Dims& d,

@@ -92,6 +92,11 @@ struct ZqArgs {
     // wavelet[step].
     const BatchShot* batch;
     int rot, step;
+    // Ragged batched kWaveFwi: workgroup b steps work item items[b] of shot
+    // rshots[items[b].shot], with that shot's dims, tiling and scalars (same
+    // rotation and wavelet index as above).
+    const RaggedItem* items;
+    const RaggedShot* rshots;
     // Sweep direction: 1 = the workgroups walk z downwards (the kernel's plane
     // z is memory plane nz-1-z; src_z / rec_z arrive already mirrored).  The
     // step is symmetric in z, so both directions give bit-identical results;
@@ -230,8 +235,32 @@ void zq_kernel(ZqArgs a) {
     constexpr bool kAdj = (MODE == kAdjoint);
     constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact || kFwd || kAdj);
     static_assert(!(kFwd || kAdj) || FMT == dev::kF32, "FWI modes are FP32");
+    int item_tx = -1, item_ty = 0, item_ch = 0;  // ragged work item (tile, chunk)
     if constexpr (kFwd) {
-        if (a.batch) {  // uniform per workgroup
+        if (a.items) {  // uniform per workgroup
+            const RaggedItem it = a.items[blockIdx.x];
+            if (it.shot < 0) return;
+            const RaggedShot* sh = a.rshots + it.shot;
+            const int r = a.rot % 3;
+            a.prev = sh->ring[r];
+            a.cur = sh->ring[r == 2 ? 0 : r + 1];
+            a.next = sh->ring[r == 0 ? 2 : r - 1];
+            a.vel2 = sh->vel2;
+            a.src_z = sh->src_z;
+            a.src_y = sh->src_y;
+            a.src_x = sh->src_x;
+            a.src_val = sh->wavelet ? sh->wavelet + a.step : nullptr;
+            a.lap_out = nullptr;
+            a.nx = sh->nx;
+            a.ny = sh->ny;
+            a.nz = sh->nz;
+            a.chunk = sh->chunk;
+            a.s = sh->s;
+            if (a.mirror) a.src_z = a.nz - 1 - a.src_z;
+            item_tx = it.tile_x;
+            item_ty = it.tile_y;
+            item_ch = it.chunk_id;
+        } else if (a.batch) {  // uniform per workgroup
             // Fields read straight from the table (a private copy of the entry
             // indexed by rot would live in scratch memory).
             const BatchShot* sh = a.batch + blockIdx.y;
@@ -262,11 +291,14 @@ void zq_kernel(ZqArgs a) {
     constexpr int kHaloPer = (kHalo + kThreads - 1) / kThreads;
     __shared__ float4 lds[2][kRows][kCols];
 
-    const unsigned logical = dev::xcd_remap(blockIdx.x, gridDim.x, a.xcd_remap != 0);
-    const int tile_x = static_cast<int>(logical % static_cast<unsigned>(a.tiles_x));
-    const unsigned rest = logical / static_cast<unsigned>(a.tiles_x);
-    const int tile_y = static_cast<int>(rest % static_cast<unsigned>(a.tiles_y));
-    const int chunk_id = static_cast<int>(rest / static_cast<unsigned>(a.tiles_y));
+    int tile_x = item_tx, tile_y = item_ty, chunk_id = item_ch;
+    if (item_tx < 0) {
+        const unsigned logical = dev::xcd_remap(blockIdx.x, gridDim.x, a.xcd_remap != 0);
+        tile_x = static_cast<int>(logical % static_cast<unsigned>(a.tiles_x));
+        const unsigned rest = logical / static_cast<unsigned>(a.tiles_x);
+        tile_y = static_cast<int>(rest % static_cast<unsigned>(a.tiles_y));
+        chunk_id = static_cast<int>(rest / static_cast<unsigned>(a.tiles_y));
+    }
 
     const int tid = static_cast<int>(threadIdx.x);
     const int tx = tid % TXV;
@@ -965,6 +997,21 @@ hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int st
     a.s = s;
     shape.nt = 1;
     return launch_zq(a, d, shape, kWaveFwi, st, dev::kF32, n_shots);
+}
+
+hipError_t ragged_zq_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int rot, int step,
+                             int txv, int ty, bool nt, hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    KernelFn fn = pick(txv, ty, kWaveFwi, nt);
+    if (!fn || !shots || !items) return hipErrorInvalidValue;
+    ZqArgs a{};
+    a.items = items;
+    a.rshots = shots;
+    a.rot = rot;
+    a.step = step;
+    a.mirror = sweep_alternation() ? (step & 1) : 0;  // every shot chains step after step
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(n_items)), dim3(txv * ty), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,

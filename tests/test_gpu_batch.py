@@ -1,7 +1,9 @@
 """Batched multi-shot forward modelling on the GPU: every shot of a ragged
 batch equals the oracle's single-shot run bit for bit (default, tuned and every
-built launch shape), reruns of the captured graph are identical, and bad shots
-are rejected."""
+launch plan: each built tile shape with short, medium and whole-column
+z-chunks, with and without the per-XCD runs; and the grouped mode at every
+built shape), reruns of the captured graph are identical, and bad shots are
+rejected."""
 from __future__ import annotations
 
 import numpy as np
@@ -37,10 +39,20 @@ def _shots(ss, dev, n_steps, seed=0, with_source=True):
     return shots, host
 
 
-def test_batch_matches_oracle_per_shot(ss, dev, oracle):
+@pytest.fixture(params=["ragged", "grouped"])
+def mode(request, monkeypatch):
+    if request.param == "grouped":
+        monkeypatch.setenv("STENCIL_BATCH_MODE", "grouped")
+    else:
+        monkeypatch.delenv("STENCIL_BATCH_MODE", raising=False)
+    return request.param
+
+
+def test_batch_matches_oracle_per_shot(ss, dev, oracle, mode):
     n = 30
     shots, host = _shots(ss, dev, n)
     b = ss.ShotBatch(shots, n, dev, tune=False)
+    assert b.ragged == (mode == "ragged")
     b.run()
     for i, (m, w, src) in enumerate(host):
         want = oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w)
@@ -52,7 +64,7 @@ def test_batch_matches_oracle_per_shot(ss, dev, oracle):
     b.close()
 
 
-def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev):
+def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev, mode):
     n = 25
     shots, _ = _shots(ss, dev, n, seed=3)
     b1 = ss.ShotBatch(shots, n, dev, tune=False)
@@ -64,7 +76,11 @@ def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev):
     b1.close()
     ss.clear_tuning()
     b2 = ss.ShotBatch(shots, n, dev, tune=True)
-    assert all(ss.tuned_shape(s["grid"])[1] for s in shots)
+    if mode == "grouped":
+        assert all(ss.tuned_shape(s["grid"])[1] for s in shots)
+    else:  # one tuned tile shape for the whole batch
+        tiles = {(g["shape"]["tile_x"], g["shape"]["tile_y"]) for g in b2.groups()}
+        assert len(tiles) == 1
     b2.run()
     for i in range(len(shots)):
         assert bool((b2.newest(i) == first[i]).all()), i
@@ -72,15 +88,49 @@ def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev):
     ss.clear_tuning()
 
 
-def test_batch_every_tile_shape_matches_oracle(ss, dev, oracle):
-    """The batched forward kernel at every built tile shape (1 to 3 z-chunks,
-    both workgroup orders), given to the groups through the tuner cache:
-    every shot equals the oracle's single-shot run bit for bit."""
+TILES = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
+
+
+def test_ragged_batch_every_plan_matches_oracle(ss, dev, oracle, monkeypatch):
+    """The one-launch-per-step batch at every built tile shape, with z-chunks
+    of 9 planes (several chunks per column, ragged last chunk), 18 and whole
+    columns, with and without the per-XCD runs: every shot equals the
+    oracle's single-shot run bit for bit."""
+    monkeypatch.delenv("STENCIL_BATCH_MODE", raising=False)
     n = 12
     shots, host = _shots(ss, dev, n, seed=5)
     want = [oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w) for m, w, src in host]
-    tiles = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
-    for j, (tx4, ty) in enumerate(tiles):
+    b = ss.ShotBatch(shots, n, dev, tune=False)
+    assert b.ragged
+    for tx4, ty in TILES:
+        for chunk in (9, 18, 1000):
+            for runs in (True, False):
+                b.set_plan(4 * tx4, ty, chunk, runs)
+                gs = b.groups()
+                assert all(g["shape"]["tile_x"] == 4 * tx4 and g["shape"]["tile_y"] == ty for g in gs)
+                if chunk == 1000:
+                    assert all(g["shape"]["n_chunks"] == 1 for g in gs)
+                b.run()
+                for i in range(len(shots)):
+                    assert np.array_equal(b.newest(i).cpu().numpy(), want[i]), (tx4, ty, chunk, runs, i)
+    b.close()
+    with pytest.raises(ss.StencilError):
+        b2 = ss.ShotBatch(shots, n, dev, tune=False)
+        try:
+            b2.set_plan(4 * 24, 8, 18)  # not a built shape
+        finally:
+            b2.close()
+
+
+def test_batch_every_tile_shape_matches_oracle(ss, dev, oracle, monkeypatch):
+    """The grouped mode's batched forward kernel at every built tile shape (1
+    to 3 z-chunks, both workgroup orders), given to the groups through the
+    tuner cache: every shot equals the oracle's single-shot run bit for bit."""
+    monkeypatch.setenv("STENCIL_BATCH_MODE", "grouped")
+    n = 12
+    shots, host = _shots(ss, dev, n, seed=5)
+    want = [oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w) for m, w, src in host]
+    for j, (tx4, ty) in enumerate(TILES):
         for chunks in (1, 3):
             ss.clear_tuning()
             shp = {"tile_x": 4 * tx4, "tile_y": ty, "n_chunks": chunks, "xcd_remap": (j + chunks) % 2,
@@ -88,6 +138,7 @@ def test_batch_every_tile_shape_matches_oracle(ss, dev, oracle):
             for s in shots:
                 ss.set_tuned_shape(s["grid"], "wave", shp)
             b = ss.ShotBatch(shots, n, dev, tune=False)
+            assert not b.ragged
             assert all(g["shape"] == shp for g in b.groups())
             b.run()
             for i in range(len(shots)):
@@ -96,7 +147,7 @@ def test_batch_every_tile_shape_matches_oracle(ss, dev, oracle):
     ss.clear_tuning()
 
 
-def test_shot_without_source_stays_zero(ss, dev):
+def test_shot_without_source_stays_zero(ss, dev, mode):
     n = 12
     shots, _ = _shots(ss, dev, n, seed=5, with_source=False)
     b = ss.ShotBatch(shots, n, dev, tune=False)
