@@ -3,19 +3,30 @@
 /**
  * include/stencil/config.hpp -- runtime configuration and CLI parser.
  *
- * Same fields, defaults, flags and errors as the reference's
- * include/stencil/config.hpp:19-191.  Additions (none change the reference
- * behaviour when unused): an explicit time step `dt` (0 = derive it from the
- * CFL limit as the reference does), seeded velocity models / initial
- * conditions for the MI355X workloads, and `--dt`, `--vmodel`, `--ic`,
- * `--seed` flags.
+ * The Config fields, their defaults and the command-line flags are the
+ * reference's interface (include/stencil/config.hpp:19-191), so callers and
+ * scripts written against it work unchanged: the grid (--nx/--ny/--nz,
+ * --dx/--dy/--dz; --ny/--nz must then equal --nx), --steps, --warmup,
+ * --velocity, --diffusivity, --physics wave|heat, --kernel
+ * naive|tiled|multigpu, --device, --output, --verbose.  parse_args throws
+ * std::runtime_error for an unknown flag, a missing or malformed value, an
+ * unknown physics / kernel name and a non-cubic --ny/--nz.  Numbers are
+ * parsed strictly (the whole argument, non-negative integers for counts).
+ * Additions (none change the reference behaviour when unused): an explicit
+ * time step `dt` (0 = derive it from the CFL limit as the reference does),
+ * seeded velocity models / initial conditions for the MI355X workloads, and
+ * the --dt, --vmodel, --ic, --seed flags.
  */
 
 #include <algorithm>
+#include <cerrno>
+#include <cmath>
 #include <cstdint>
-#include <cstring>
+#include <cstdlib>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "stencil/types.hpp"
 
@@ -104,106 +115,130 @@ struct BasicConfig {
 };
 using Config = BasicConfig<real_t>;
 
-/**
- * parse_args -- reference config.hpp:108-191: same flags, same exceptions
- * (std::runtime_error on unknown flags / values and on non-cubic --ny/--nz).
- */
+namespace detail {
+
+inline std::runtime_error bad_value(const std::string& flag, const std::string& v, const char* what) {
+    return std::runtime_error("parse_args: " + flag + " expects " + what + ", got '" + v + "'");
+}
+
+/// A count: decimal digits only, fits in 64 bits.
+inline std::uint64_t parse_count(const std::string& flag, const std::string& v) {
+    if (v.empty() || v.find_first_not_of("0123456789") != std::string::npos)
+        throw bad_value(flag, v, "a non-negative integer");
+    errno = 0;
+    char* end = nullptr;
+    const unsigned long long x = std::strtoull(v.c_str(), &end, 10);
+    if (errno == ERANGE || end != v.c_str() + v.size()) throw bad_value(flag, v, "a non-negative integer");
+    return static_cast<std::uint64_t>(x);
+}
+
+inline int parse_small(const std::string& flag, const std::string& v) {
+    const std::uint64_t x = parse_count(flag, v);
+    if (x > 0x7fffffffu) throw bad_value(flag, v, "an integer below 2^31");
+    return static_cast<int>(x);
+}
+
+/// A finite real number, the whole argument.
+inline double parse_real(const std::string& flag, const std::string& v) {
+    errno = 0;
+    char* end = nullptr;
+    const double x = std::strtod(v.c_str(), &end);
+    if (v.empty() || errno == ERANGE || end != v.c_str() + v.size() || !std::isfinite(x))
+        throw bad_value(flag, v, "a finite number");
+    return x;
+}
+
+/// One of a fixed set of names.
+template <class E>
+E parse_choice(const std::string& flag, const std::string& v,
+               std::initializer_list<std::pair<const char*, E>> names) {
+    std::string known;
+    for (const auto& n : names) {
+        if (v == n.first) return n.second;
+        known += known.empty() ? n.first : std::string("|") + n.first;
+    }
+    throw std::runtime_error("parse_args: " + flag + " '" + v + "' is not one of " + known);
+}
+
+/// A flag and what its value does to the config (value-less flags get "").
+template <class T>
+struct FlagRule {
+    const char* name;
+    bool has_value;
+    void (*apply)(BasicConfig<T>&, const std::string&);
+};
+
+template <class T>
+const FlagRule<T>* find_rule(const std::string& flag) {
+    using C = BasicConfig<T>;
+    static const FlagRule<T> rules[] = {
+        {"--nx", true, [](C& c, const std::string& v) { c.nx = parse_count("--nx", v); }},
+        {"--ny", true, [](C& c, const std::string& v) { c.ny = parse_count("--ny", v); c.ny_explicit = true; }},
+        {"--nz", true, [](C& c, const std::string& v) { c.nz = parse_count("--nz", v); c.nz_explicit = true; }},
+        {"--dx", true, [](C& c, const std::string& v) { c.dx = static_cast<T>(parse_real("--dx", v)); }},
+        {"--dy", true, [](C& c, const std::string& v) { c.dy = static_cast<T>(parse_real("--dy", v)); }},
+        {"--dz", true, [](C& c, const std::string& v) { c.dz = static_cast<T>(parse_real("--dz", v)); }},
+        {"--dt", true, [](C& c, const std::string& v) { c.dt = static_cast<T>(parse_real("--dt", v)); }},
+        {"--steps", true, [](C& c, const std::string& v) { c.steps = parse_small("--steps", v); }},
+        {"--warmup", true, [](C& c, const std::string& v) { c.warmup_steps = parse_small("--warmup", v); }},
+        {"--velocity", true, [](C& c, const std::string& v) { c.velocity = static_cast<T>(parse_real("--velocity", v)); }},
+        {"--diffusivity", true,
+         [](C& c, const std::string& v) { c.diffusivity = static_cast<T>(parse_real("--diffusivity", v)); }},
+        {"--seed", true, [](C& c, const std::string& v) { c.seed = parse_count("--seed", v); }},
+        {"--device", true, [](C& c, const std::string& v) { c.device = parse_small("--device", v); }},
+        {"--output", true, [](C& c, const std::string& v) { c.output_path = v; }},
+        {"--verbose", false, [](C& c, const std::string&) { c.verbose = true; }},
+        {"--physics", true,
+         [](C& c, const std::string& v) {
+             c.physics = parse_choice<Physics>("--physics", v,
+                                               {{"wave", Physics::WavePropagation}, {"heat", Physics::HeatDiffusion}});
+         }},
+        {"--kernel", true,
+         [](C& c, const std::string& v) {
+             c.variant = parse_choice<KernelVariant>(
+                 "--kernel", v,
+                 {{"naive", KernelVariant::Naive}, {"tiled", KernelVariant::Tiled}, {"multigpu", KernelVariant::MultiGPU}});
+         }},
+        {"--vmodel", true,
+         [](C& c, const std::string& v) {
+             c.vmodel = parse_choice<VelocityModel>(
+                 "--vmodel", v, {{"constant", VelocityModel::Constant}, {"layered-noise", VelocityModel::LayeredNoise}});
+         }},
+        {"--ic", true,
+         [](C& c, const std::string& v) {
+             c.ic = parse_choice<InitialCondition>(
+                 "--ic", v, {{"gaussian", InitialCondition::Gaussian}, {"noise-cube", InitialCondition::NoiseCube}});
+         }},
+    };
+    for (const auto& r : rules)
+        if (flag == r.name) return &r;
+    return nullptr;
+}
+
+}  // namespace detail
+
+/// Fills a config from argv (argv[0] is the program name).
 template <class T>
 inline BasicConfig<T> parse_args_t(int argc, char** argv) {
     BasicConfig<T> cfg;
-    for (int i = 1; i < argc; ++i) {
-        auto eq = [&](const char* flag) { return std::strcmp(argv[i], flag) == 0; };
-        auto next_int = [&]() -> std::size_t {
-            if (++i >= argc) throw std::runtime_error("missing value for flag");
-            return static_cast<std::size_t>(std::stoul(argv[i]));
-        };
-        auto next_float = [&]() -> T {
-            if (++i >= argc) throw std::runtime_error("missing value for flag");
-            return static_cast<T>(std::stod(argv[i]));
-        };
-        auto next_str = [&]() -> std::string {
-            if (++i >= argc) throw std::runtime_error("missing value for flag");
-            return argv[i];
-        };
-
-        if (eq("--nx"))
-            cfg.nx = next_int();
-        else if (eq("--ny")) {
-            cfg.ny = next_int();
-            cfg.ny_explicit = true;
-        } else if (eq("--nz")) {
-            cfg.nz = next_int();
-            cfg.nz_explicit = true;
-        } else if (eq("--dx"))
-            cfg.dx = next_float();
-        else if (eq("--dy"))
-            cfg.dy = next_float();
-        else if (eq("--dz"))
-            cfg.dz = next_float();
-        else if (eq("--steps"))
-            cfg.steps = static_cast<int>(next_int());
-        else if (eq("--warmup"))
-            cfg.warmup_steps = static_cast<int>(next_int());
-        else if (eq("--velocity"))
-            cfg.velocity = next_float();
-        else if (eq("--diffusivity"))
-            cfg.diffusivity = next_float();
-        else if (eq("--dt"))
-            cfg.dt = next_float();
-        else if (eq("--seed"))
-            cfg.seed = static_cast<std::uint64_t>(next_int());
-        else if (eq("--device"))
-            cfg.device = static_cast<int>(next_int());
-        else if (eq("--output"))
-            cfg.output_path = next_str();
-        else if (eq("--verbose"))
-            cfg.verbose = true;
-        else if (eq("--physics")) {
-            const std::string v = next_str();
-            if (v == "wave")
-                cfg.physics = Physics::WavePropagation;
-            else if (v == "heat")
-                cfg.physics = Physics::HeatDiffusion;
-            else
-                throw std::runtime_error("unknown physics: " + v);
-        } else if (eq("--kernel")) {
-            const std::string v = next_str();
-            if (v == "naive")
-                cfg.variant = KernelVariant::Naive;
-            else if (v == "tiled")
-                cfg.variant = KernelVariant::Tiled;
-            else if (v == "multigpu")
-                cfg.variant = KernelVariant::MultiGPU;
-            else
-                throw std::runtime_error("unknown kernel: " + v);
-        } else if (eq("--vmodel")) {
-            const std::string v = next_str();
-            if (v == "constant")
-                cfg.vmodel = VelocityModel::Constant;
-            else if (v == "layered-noise")
-                cfg.vmodel = VelocityModel::LayeredNoise;
-            else
-                throw std::runtime_error("unknown velocity model: " + v);
-        } else if (eq("--ic")) {
-            const std::string v = next_str();
-            if (v == "gaussian")
-                cfg.ic = InitialCondition::Gaussian;
-            else if (v == "noise-cube")
-                cfg.ic = InitialCondition::NoiseCube;
-            else
-                throw std::runtime_error("unknown initial condition: " + v);
-        } else {
-            throw std::runtime_error(std::string("unknown flag: ") + argv[i]);
+    int i = 1;
+    while (i < argc) {
+        const std::string flag = argv[i++];
+        const detail::FlagRule<T>* rule = detail::find_rule<T>(flag);
+        if (!rule) throw std::runtime_error("parse_args: unrecognised option '" + flag + "'");
+        std::string value;
+        if (rule->has_value) {
+            if (i >= argc) throw std::runtime_error("parse_args: option " + flag + " needs a value");
+            value = argv[i++];
         }
+        rule->apply(cfg, value);
     }
-
-    if ((cfg.ny_explicit || cfg.nz_explicit) && (cfg.nx != cfg.ny || cfg.ny != cfg.nz)) {
-        throw std::runtime_error("Non-cubic grids aren't supported yet (got --nx " +
-                                 std::to_string(cfg.nx) + " --ny " + std::to_string(cfg.ny) +
-                                 " --nz " + std::to_string(cfg.nz) +
-                                 ") -- pass --nx alone for an N×N×N cube, or make --ny/--nz "
-                                 "match it.");
-    }
+    // The solvers take N^3 grids: --nx alone selects one; explicit --ny/--nz must agree.
+    const bool cubic = cfg.nx == cfg.ny && cfg.ny == cfg.nz;
+    if ((cfg.ny_explicit || cfg.nz_explicit) && !cubic)
+        throw std::runtime_error("parse_args: grids must be cubic, got nx " + std::to_string(cfg.nx) + ", ny " +
+                                 std::to_string(cfg.ny) + ", nz " + std::to_string(cfg.nz) +
+                                 " (give --nx alone, or --ny / --nz equal to it)");
     return cfg;
 }
 

@@ -11,6 +11,7 @@
  * both.
  */
 
+#include <algorithm>
 #include <memory>
 #include <string>
 
@@ -31,6 +32,15 @@ public:
     virtual void download(BasicHostGrid<T>& out) const = 0;
     virtual const BasicGridDims<T>& dims() const = 0;
     virtual std::string name() const = 0;
+    /// Addition: z-plane z of the newest level (nx*ny values, x fastest) into
+    /// out, synchronised.  This default downloads the whole field; the
+    /// library's solvers copy the one contiguous plane instead.
+    virtual void download_plane_z(std::size_t z, T* out) const {
+        const BasicGridDims<T>& d = dims();
+        BasicHostGrid<T> h(d);
+        download(h);
+        std::copy_n(h.ptr() + d.idx(0, 0, z), d.nx * d.ny, out);
+    }
 };
 using ISolver = BasicISolver<real_t>;
 

@@ -1,52 +1,40 @@
-// stencil_naive: the coalesced global-memory kernel over one grid or the
-// 128/256/512 sweep (same flags, output and JSON as the reference's
-// src/main_naive.cpp).
-#include <cstdio>
+// stencil_naive: times the coalesced global-memory kernel on one grid or on
+// the 128/256/512 sweep and writes the results file (flags and JSON schema of
+// the reference's stencil_naive, src/main_naive.cpp).
 #include <cstdlib>
-#include <stdexcept>
+#include <exception>
 #include <vector>
 
 #include "common.hpp"
-#include "stencil/config.hpp"
-#include "stencil/io.hpp"
-#include "stencil/solver.hpp"
-#include "stencil/timer.hpp"
 
 int main(int argc, char** argv) {
-    std::printf("=== stencil-solver: naive kernel ===\n\n");
-    stencil::Config cfg;
+    using namespace stencil;
+    Config cfg;
     try {
-        cfg = stencil::parse_args(argc, argv);
+        cfg = parse_args(argc, argv);
     } catch (const std::exception& e) {
-        std::fprintf(stderr,
-                     "Error: %s\n\nUsage: %s [--nx N] [--ny N] [--nz N] [--steps N] [--warmup N] "
-                     "[--physics wave|heat] [--velocity F] [--device N] [--output path] [--verbose]\n",
-                     e.what(), argv[0]);
-        return EXIT_FAILURE;
+        return app::refuse(argv[0], e.what(),
+                           "[--nx N] [--steps N] [--warmup N] [--physics wave|heat] [--velocity V] "
+                           "[--diffusivity D] [--device N] [--output FILE.json] [--verbose]");
     }
-    cfg.variant = stencil::KernelVariant::Naive;  // this program always runs the global-memory kernel
-    stencil::app::print_device_info(cfg.device);
-    std::printf("\n");
-
-    std::vector<stencil::PerfResult> results;
-    for (std::size_t n : stencil::app::sweep_sizes(cfg.nx, cfg.ny, cfg.nz)) {
-        stencil::Config run = cfg;
-        run.nx = run.ny = run.nz = n;
-        std::printf("Grid: %zu^3   steps: %d   warmup: %d\n", n, run.steps, run.warmup_steps);
+    app::describe_device(cfg.device);
+    std::vector<PerfResult> rows;
+    for (std::size_t n : app::grid_sizes(cfg)) {
         try {
-            auto solver = stencil::make_solver(run);
-            stencil::PerfResult r = solver->run(run.steps, run.warmup_steps);
-            stencil::print_perf(r);
-            results.push_back(r);
+            rows.push_back(app::timed_run(cfg, n, KernelVariant::Naive));
         } catch (const std::exception& e) {
-            std::fprintf(stderr, "  FAILED: %s\n", e.what());
+            std::fprintf(stderr, "stencil_naive: %zu^3 run failed: %s\n", n, e.what());
             return EXIT_FAILURE;
         }
+        std::printf("%zu^3, %d steps after %d warm-up steps:\n", n, cfg.steps, cfg.warmup_steps);
+        print_perf(rows.back());
     }
-    if (results.size() > 1) {
-        std::printf("\n-- Summary --\n");
-        stencil::format_perf_table(results);
+    format_perf_table(rows);
+    try {
+        write_results_json(cfg.output_path, rows, cfg.device, cfg.steps, cfg.warmup_steps);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "stencil_naive: %s\n", e.what());
+        return EXIT_FAILURE;
     }
-    stencil::write_results_json(cfg.output_path, results, cfg.device, cfg.steps, cfg.warmup_steps);
     return EXIT_SUCCESS;
 }

@@ -102,6 +102,16 @@ public:
 
     const BasicGridDims<T>& dims() const override { return grid_->dims(); }
 
+    /// One plane of the newest level: the plane is contiguous in the
+    /// [nz][ny][nx] layout, so this is a single nx*ny copy.
+    void download_plane_z(std::size_t z, T* out) const override {
+        const auto& d = grid_->dims();
+        if (z >= d.nz) throw std::out_of_range("download_plane_z: plane " + std::to_string(z) + " outside the grid");
+        const std::size_t plane = d.nx * d.ny;
+        HIP_CHECK(hipMemcpyAsync(out, grid_->cur() + z * plane, plane * sizeof(T), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
     hipStream_t stream() const { return stream_; }
 
 protected:

@@ -59,6 +59,14 @@ def ring_run(ref: RefLib, p0, vel2, dx, dt, n_steps):
     return bufs[(n_steps + 1) % 3]
 
 
+# stencil_slice runs recorded from the reference program (flags as passed).
+SLICE_RUNS = {
+    "slice_wave_32": ["--nx", "32", "--steps", "20", "--every", "5", "--kernel", "naive"],
+    "slice_heat_24": ["--nx", "24", "--steps", "12", "--every", "4", "--physics", "heat", "--kernel", "tiled",
+                      "--diffusivity", "0.02"],
+}
+
+
 def main():
     nofma, native, f64 = RefLib("nofma"), RefLib("native"), RefLib("f64")
     meta: dict = {"builds": {"nofma": "-O3 -march=x86-64-v3 -ffp-contract=off",
@@ -149,6 +157,21 @@ def main():
         b = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
         meta["norms"].append({"seed": seed, "shape": list(shape), "l2_error": nofma.l2_error(a, b),
                               "max_abs_error": nofma.max_abs_error(a, b)})
+
+    # 8. The reference's stencil_slice program (src/main_slice.cpp, built by
+    #    oracle/Makefile as oracle/_ref/ref_slice): its .f32 frames and .json
+    #    metadata, kept as written.
+    import subprocess
+    import tempfile
+    prog = ROOT / "oracle" / "_ref" / "ref_slice"
+    meta["slices"] = {}
+    for name, args in SLICE_RUNS.items():
+        with tempfile.TemporaryDirectory() as tmp:
+            prefix = Path(tmp) / name
+            subprocess.run([str(prog), *args, "--slice-out", str(prefix)], check=True, capture_output=True)
+            (OUT / f"{name}.f32").write_bytes(prefix.with_suffix(".f32").read_bytes())
+            (OUT / f"{name}.json").write_text(prefix.with_suffix(".json").read_text())
+        meta["slices"][name] = {"args": args}
 
     (OUT / "golden_meta.json").write_text(json.dumps(meta, indent=1, sort_keys=True))
     print("wrote", sorted(p.name for p in OUT.glob("*.npz")))
