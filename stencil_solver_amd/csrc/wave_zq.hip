@@ -646,8 +646,13 @@ void zq_kernel(ZqArgs a) {
     } else {
         // 16-bit storage: every load is issued one step ahead as raw 16-bit
         // words (the queue refill for plane zz+5, halo and streams for plane
-        // zz+1) and decoded at the top of the step that uses it, so the raw
-        // registers are free again before they are reloaded.
+        // zz+1) and decoded in the step that uses it, just before it is
+        // reloaded.  Before the plane's barrier a step does only what LDS
+        // needs (decode and store the halo, reload it); the queue's newest
+        // plane and the p_prev / vel2 streams are decoded and reloaded after
+        // the barrier, so the barrier phase is short and the waves holding
+        // halo items do not hold the others up (+6 % block-FP16, +7 % bf16 at
+        // 512^3 against decoding everything before the barrier).
         // Halo items exist for the first kHalo threads (whole waves but one),
         // so the others skip the decodes.  The loads stay unconditional: under
         // a divergent branch the compiler stops treating the plane descriptors
@@ -673,11 +678,24 @@ void zq_kernel(ZqArgs a) {
                 const int zz = zb + j;
                 const int z1 = min(zz + 1, z_last);
                 const int z5 = min(zz + kR + 1, z_last);
-                q[(j + 8) % 9] = decode<FMT, kExact>(rq, rq_e, p_inv);
+                // before the barrier only what LDS needs: the halo of plane zz
                 float4 h[kHaloPer];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
                     if (h_use[k]) h[k] = decode<FMT, kExact>(rh[k], rh_e[k], p_inv);
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k) {
+                    rh[k] = raw_at(cur_r, z1, h_ld[k]);
+                    rh_e[k] = exp_at(cur_er, z1, h_eld[k]);
+                }
+                const int b = (zz - z_begin) & 1;
+                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (h_use[k]) lds[b][h_row[k]][h_col[k]] = h[k];
+                __syncthreads();
+                // after it: the queue's newest plane and the streams of plane zz
+                q[(j + 8) % 9] = decode<FMT, kExact>(rq, rq_e, p_inv);
                 const float4 pv = decode<FMT, kExact>(rp, rp_e, p_inv);
                 float4 vv;
                 float dt2v = a.s.dt2;
@@ -690,21 +708,10 @@ void zq_kernel(ZqArgs a) {
                 }
                 rq = raw_at(cur_r, z5, own_ld);
                 rq_e = exp_at(cur_er, z5, own_eld);
-#pragma unroll
-                for (int k = 0; k < kHaloPer; ++k) {
-                    rh[k] = raw_at(cur_r, z1, h_ld[k]);
-                    rh_e[k] = exp_at(cur_er, z1, h_eld[k]);
-                }
                 rp = raw_stream(prev_r, z1);
                 rv = raw_stream(vel_r, z1);
                 rp_e = exp_at(prev_er, z1, own_eld);
                 rv_e = exp_at(vel_er, z1, own_eld);
-                const int b = (zz - z_begin) & 1;
-                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
-#pragma unroll
-                for (int k = 0; k < kHaloPer; ++k)
-                    if (h_use[k]) lds[b][h_row[k]][h_col[k]] = h[k];
-                __syncthreads();
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
