@@ -453,7 +453,9 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
     from {96, 128, 160, 192, 256, 320} (seed 4); each shot: seeded smooth
     velocity (Gaussian-filtered noise, sigma 8 cells) mapped to [1500, 4500]
     m/s, h = 10 m, dt = 8e-4 s, 15 Hz Ricker at a random interior cell; FP32.
-    Launch shapes autotuned per grid size; the run is one hipGraph launch."""
+    Launch shapes autotuned per grid size; the run is one hipGraph launch.
+    Then the same run as one persistent launch (STENCIL_BATCH_MODE=persistent),
+    timed the same way and compared shot by shot for bit-equality."""
     import numpy as np
     import torch
     from stencil_solver_amd import synth
@@ -497,7 +499,34 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
            "groups": b.groups()}
     print(f"[bench] batch {n_shots} shots x {steps}: {ms:.0f} ms, {gpts:.1f} Gpts/s, {gbs:.0f} GB/s "
           f"(graph build {t_build:.1f}s)", file=sys.stderr, flush=True)
-    b.close()
+    if b.ragged and os.environ.get("STENCIL_BATCH_MODE") is None:
+        # the same run as ONE persistent launch (queues + grid barrier), for comparison
+        newest = [b.newest(i).clone() for i in range(n_shots)]
+        b.close()
+        os.environ["STENCIL_BATCH_MODE"] = "persistent"
+        try:
+            bp = ss.ShotBatch(shots, steps, dev, tune=tune)
+        finally:
+            del os.environ["STENCIL_BATCH_MODE"]
+        bp.run()
+        bp.check()
+        e0.record()
+        bp.run()
+        e1.record()
+        torch.cuda.synchronize()
+        bp.check()
+        pms = e0.elapsed_time(e1)
+        same = all(bool((bp.newest(i) == newest[i]).all()) for i in range(n_shots))
+        out["persistent"] = {"run_ms": round(pms, 1), "workgroups": bp.persistent_grid,
+                             "roofline_frac": round(16.0 * pts * steps / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "vs_per_step_launches": round(ms / pms, 3), "bit_equal": same,
+                             "groups": bp.groups()}
+        print(f"[bench] batch as one persistent launch: {pms:.0f} ms ({ms / pms:.3f}x), bit-equal {same}",
+              file=sys.stderr, flush=True)
+        bp.close()
+        del bp, newest
+    else:
+        b.close()
     del b, shots
     torch.cuda.empty_cache()
     return out

@@ -50,7 +50,8 @@ enum stencil_status {
     STENCIL_ERR_INVALID = 1,     /* bad dims / null pointer / bad enum */
     STENCIL_ERR_HIP = 2,         /* a HIP runtime call failed */
     STENCIL_ERR_UNSUPPORTED = 3, /* shape the kernel cannot take */
-    STENCIL_ERR_NOMEM = 4
+    STENCIL_ERR_NOMEM = 4,
+    STENCIL_ERR_TIMEOUT = 5      /* a device-side wait gave up (results invalid) */
 };
 
 const char* stencil_strerror(int status);
@@ -390,8 +391,19 @@ int stencil_batch_group(const stencil_batch* batch, int group, stencil_grid* gri
  * grouped batch.  Synchronises. */
 int stencil_batch_set_plan(stencil_batch* batch, const stencil_tile_shape* shape, int chunk_planes,
                            void* hip_stream);
-/* 1 for a ragged batch (one launch per step), 0 for a grouped one. */
+/* 1 for a ragged batch (one launch per step, or one persistent launch for
+ * the run), 0 for a grouped one. */
 int stencil_batch_is_ragged(const stencil_batch* batch);
+/* Workgroups of the persistent launch (STENCIL_BATCH_MODE=persistent: one
+ * launch of workgroups that are all resident together steps every shot
+ * n_steps times, taking each step's work items from per-XCD queues and
+ * meeting at a grid barrier between steps); 0 when the run is one launch per
+ * step. */
+int stencil_batch_persistent_grid(const stencil_batch* batch);
+/* Waits for the stream, then STENCIL_OK, or STENCIL_ERR_TIMEOUT when a grid
+ * barrier of the last persistent run gave up (its fields are then not
+ * valid). */
+int stencil_batch_check(const stencil_batch* batch, void* hip_stream);
 void stencil_batch_destroy(stencil_batch* batch);
 
 /* ---- Solver handles: ISolver (reference include/stencil/solver.hpp:13-24)

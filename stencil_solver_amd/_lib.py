@@ -132,6 +132,8 @@ SIGNATURES = {
     "stencil_batch_group": (_I, [_P, _I, _G, C.POINTER(stencil_tile_shape), C.POINTER(_I)]),
     "stencil_batch_set_plan": (_I, [_P, C.POINTER(stencil_tile_shape), _I, _P]),
     "stencil_batch_is_ragged": (_I, [_P]),
+    "stencil_batch_persistent_grid": (_I, [_P]),
+    "stencil_batch_check": (_I, [_P, _P]),
     "stencil_batch_destroy": (None, [_P]),
     "stencil_config_default": (None, [C.POINTER(stencil_config)]),
     "stencil_solver_create": (_I, [C.POINTER(stencil_config), C.POINTER(_P)]),

@@ -63,11 +63,26 @@ class ShotBatch:
         """All shots, n_steps each, from zero fields: one graph launch."""
         check(lib().stencil_batch_run(self._h, _stream(stream)), "batch_run")
 
+    def check(self, stream=None) -> None:
+        """Waits for the stream; raises StencilError when a grid barrier of
+        the last persistent run timed out (its fields are then invalid)."""
+        check(lib().stencil_batch_check(self._h, _stream(stream)), "batch_check")
+
     @property
     def ragged(self) -> bool:
         """True when every step of every shot is one launch (the default);
         False for the grouped mode (STENCIL_BATCH_MODE=grouped at creation)."""
         return lib().stencil_batch_is_ragged(self._h) == 1
+
+    @property
+    def persistent_grid(self) -> int:
+        """Workgroups of the one persistent launch that steps every shot
+        (STENCIL_BATCH_MODE=persistent at creation); 0 when the run is one
+        launch per step."""
+        r = lib().stencil_batch_persistent_grid(self._h)
+        if r < 0:
+            check(r, "batch_persistent_grid")
+        return r
 
     def set_plan(self, tile_x: int, tile_y: int, chunk_planes: int, xcd_runs: bool = True, stream=None) -> None:
         """Re-plan the ragged launch: tiles of tile_x x tile_y cells (a built

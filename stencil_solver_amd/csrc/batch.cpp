@@ -21,6 +21,12 @@
 // of every shot, then n_steps launches -- is captured once into a hipGraph;
 // a run is one graph launch.
 //
+// Persistent mode (STENCIL_BATCH_MODE=persistent, the same plan): the n_steps
+// launches are replaced by ONE launch of as many workgroups as stay resident
+// together, which take each step's items from per-XCD queues and meet at a
+// grid barrier between steps (device counters the graph zeroes first; a
+// barrier that times out aborts the run and stencil_batch_check reports it).
+//
 // Grouped mode (STENCIL_BATCH_MODE=grouped, kept for comparison): shots of
 // equal grids form a group; each group has its own tuned shape and one launch
 // per step (workgroup row = shot), the group chains being independent graph
@@ -48,6 +54,9 @@ using namespace stencil::hipk;
 namespace {
 
 // A ragged launch plan: tile shape, z-chunk length target, XCD runs on/off.
+// Words of the persistent kernel's counters: abort flag, barrier, 8 queues.
+constexpr size_t kSyncWords = 10;
+
 struct RaggedPlan {
     int txv = 16, ty = 16;
     int chunk_target = 90;
@@ -69,6 +78,9 @@ struct stencil_batch {
     std::vector<stencil_shot> shots;
     int n_steps = 0;
     bool ragged = true;
+    bool persistent = false;  // ragged plan, one cooperative launch for all steps
+    int persist_grid = 0;     // its workgroups
+    unsigned* d_sync = nullptr;  // device: abort flag, barrier counter, 8 queues
     RaggedPlan plan;
     RaggedShot* d_shots = nullptr;  // device, one per shot
     RaggedItem* d_items = nullptr;  // device, n_items
@@ -96,9 +108,9 @@ bool shot_ok(const stencil_shot& s) {
     return zq_supported(d, ptrs, 4);
 }
 
-bool batch_mode_grouped() {
+bool batch_mode_is(const char* name) {
     const char* e = std::getenv("STENCIL_BATCH_MODE");
-    return e && std::strcmp(e, "grouped") == 0;
+    return e && std::strcmp(e, name) == 0;
 }
 
 void release(stencil_batch* b) {
@@ -109,6 +121,7 @@ void release(stencil_batch* b) {
         if (g.table) (void)hipFree(g.table);
     if (b->d_shots) (void)hipFree(b->d_shots);
     if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_sync) (void)hipFree(b->d_sync);
     delete b;
 }
 
@@ -299,9 +312,18 @@ int build_graph(stencil_batch* b) {
             // p^{-1} = p^0 = 0 (and the third level, so halos stay zero): fill
             // kernels, not memset nodes
             B_TRY(zero_rings(b, st));
-            for (int n = 0; n < b->n_steps; ++n)
-                B_TRY(ragged_zq_fwi_f32(b->d_shots, b->d_items, b->n_items, n % 3, n, b->plan.txv, b->plan.ty, true,
-                                        st));
+            if (b->persistent) {
+                const int cap = ragged_persist_grid(b->plan.txv, b->plan.ty);
+                b->persist_grid = std::min(cap, (b->n_items + 7) / 8 * 8);
+                if (b->persist_grid < 8) B_TRY(hipErrorNotSupported);
+                B_TRY(fill_f32(reinterpret_cast<float*>(b->d_sync), kSyncWords, 0.f, st));  // zero bits
+                B_TRY(ragged_persist_fwi_f32(b->d_shots, b->d_items, b->n_items, b->n_steps, b->plan.txv, b->plan.ty,
+                                             b->persist_grid, b->d_sync, st));
+            } else {
+                for (int n = 0; n < b->n_steps; ++n)
+                    B_TRY(ragged_zq_fwi_f32(b->d_shots, b->d_items, b->n_items, n % 3, n, b->plan.txv, b->plan.ty,
+                                            true, st));
+            }
         } else {
             B_TRY(hipEventRecord(fork, st));
             for (size_t gi = 0; gi < b->groups.size(); ++gi) {
@@ -386,7 +408,8 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
     if (!b) return STENCIL_ERR_HIP;
     b->shots.assign(shots, shots + n_shots);
     b->n_steps = n_steps;
-    b->ragged = !batch_mode_grouped();
+    b->ragged = !batch_mode_is("grouped");
+    b->persistent = b->ragged && batch_mode_is("persistent");
     // group by grid (the grouped mode's launches; reported in either mode)
     using Key = std::tuple<uint64_t, uint64_t, uint64_t, double, double, double, double>;
     std::map<Key, int> index;
@@ -406,7 +429,9 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
         b->groups[it->second].shots.push_back(i);
     }
     int rc = STENCIL_OK;
-    if (b->ragged) {
+    if (b->persistent && hipMalloc(&b->d_sync, kSyncWords * sizeof(unsigned)) != hipSuccess) rc = STENCIL_ERR_HIP;
+    if (rc != STENCIL_OK) {
+    } else if (b->ragged) {
         const hipError_t e = tune ? tune_plan(b, 4, st) : upload_plan(b, RaggedPlan{}, st);
         if (e != hipSuccess) rc = STENCIL_ERR_HIP;
     } else {
@@ -477,6 +502,19 @@ int stencil_batch_set_plan(stencil_batch* b, const stencil_tile_shape* shape, in
 }
 
 int stencil_batch_is_ragged(const stencil_batch* b) { return b ? (b->ragged ? 1 : 0) : STENCIL_ERR_INVALID; }
+
+int stencil_batch_check(const stencil_batch* b, void* stream) {
+    if (!b) return STENCIL_ERR_INVALID;
+    if (hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess) return STENCIL_ERR_HIP;
+    if (!b->d_sync) return STENCIL_OK;
+    unsigned flag = 0;
+    if (hipMemcpy(&flag, b->d_sync, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess) return STENCIL_ERR_HIP;
+    return flag == 0 ? STENCIL_OK : STENCIL_ERR_TIMEOUT;
+}
+
+int stencil_batch_persistent_grid(const stencil_batch* b) {
+    return b ? (b->persistent ? b->persist_grid : 0) : STENCIL_ERR_INVALID;
+}
 
 void stencil_batch_destroy(stencil_batch* b) { release(b); }
 

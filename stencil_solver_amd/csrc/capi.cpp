@@ -68,6 +68,7 @@ const char* stencil_strerror(int status) {
         case STENCIL_ERR_HIP: return "HIP runtime error";
         case STENCIL_ERR_UNSUPPORTED: return "unsupported shape for this kernel";
         case STENCIL_ERR_NOMEM: return "out of device memory";
+        case STENCIL_ERR_TIMEOUT: return "a device-side wait timed out (results invalid)";
         default: return "unknown status";
     }
 }

@@ -140,6 +140,16 @@ struct RaggedItem {
 // with for this (txv, ty).
 hipError_t ragged_zq_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int rot, int step,
                              int txv, int ty, bool nt, hipStream_t st);
+// All n_steps steps of the same ragged batch in ONE launch of `grid`
+// workgroups (a multiple of 8, at most ragged_persist_grid(): all resident
+// together) that take each step's items from per-XCD queues and meet at a
+// grid barrier between steps; step n as ragged_zq_fwi_f32(..., n % 3, n, ...)
+// with non-temporal streams.  sync: 10 zeroed words (abort flag, barrier
+// counter, 8 queues); the flag is non-zero after the run when a barrier
+// timed out (the fields are then not valid).
+int ragged_persist_grid(int txv, int ty);
+hipError_t ragged_persist_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int n_steps, int txv,
+                                  int ty, int grid, unsigned* sync, hipStream_t st);
 // The forward FWI step of n_shots shots of equal dims in one launch: shot i
 // steps ring[rot] / ring[rot+1] -> ring[rot+2] (mod 3), adding wavelet[step].
 hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int step, const Dims& d,

@@ -227,9 +227,27 @@ constexpr int min_waves(int txv, int ty, int fmt, int mode) {
     return 4;
 }
 
+// A value the whole wave shares, read from the first lane, so the compiler
+// keeps it (and the buffer resources built from it) in scalar registers: the
+// work-item fields of a ragged launch are loaded by vector loads, and in the
+// persistent kernel they hang off an index that came through LDS.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+template <class T>
+__device__ __forceinline__ T* wave_uniform(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)));
+    return reinterpret_cast<T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// One workgroup's tile (work item `block`: the workgroup index of a plain
+// launch, or an index into a.items) of one step.  Returns early only with a
+// value uniform across the workgroup.
 template <int TXV, int TY, int MODE, bool NT, int FMT = dev::kF32>
-__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, FMT, MODE))))
-void zq_kernel(ZqArgs a) {
+__device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
     constexpr bool kFwd = (MODE == kWaveFwi);
     constexpr bool kAdj = (MODE == kAdjoint);
@@ -238,28 +256,33 @@ void zq_kernel(ZqArgs a) {
     int item_tx = -1, item_ty = 0, item_ch = 0;  // ragged work item (tile, chunk)
     if constexpr (kFwd) {
         if (a.items) {  // uniform per workgroup
-            const RaggedItem it = a.items[blockIdx.x];
-            if (it.shot < 0) return;
-            const RaggedShot* sh = a.rshots + it.shot;
+            const RaggedItem& it = a.items[block];
+            const int shot = wave_uniform(it.shot);
+            if (shot < 0) return;
+            const RaggedShot* sh = a.rshots + shot;
             const int r = a.rot % 3;
-            a.prev = sh->ring[r];
-            a.cur = sh->ring[r == 2 ? 0 : r + 1];
-            a.next = sh->ring[r == 0 ? 2 : r - 1];
-            a.vel2 = sh->vel2;
-            a.src_z = sh->src_z;
-            a.src_y = sh->src_y;
-            a.src_x = sh->src_x;
-            a.src_val = sh->wavelet ? sh->wavelet + a.step : nullptr;
+            a.prev = wave_uniform(sh->ring[r]);
+            a.cur = wave_uniform(sh->ring[r == 2 ? 0 : r + 1]);
+            a.next = wave_uniform(sh->ring[r == 0 ? 2 : r - 1]);
+            a.vel2 = wave_uniform(sh->vel2);
+            a.src_z = wave_uniform(sh->src_z);
+            a.src_y = wave_uniform(sh->src_y);
+            a.src_x = wave_uniform(sh->src_x);
+            const float* wavelet = wave_uniform(sh->wavelet);
+            a.src_val = wavelet ? wavelet + a.step : nullptr;
             a.lap_out = nullptr;
-            a.nx = sh->nx;
-            a.ny = sh->ny;
-            a.nz = sh->nz;
-            a.chunk = sh->chunk;
-            a.s = sh->s;
+            a.nx = wave_uniform(sh->nx);
+            a.ny = wave_uniform(sh->ny);
+            a.nz = wave_uniform(sh->nz);
+            a.chunk = wave_uniform(sh->chunk);
+            a.s.inv_dx2 = wave_uniform(sh->s.inv_dx2);
+            a.s.inv_dy2 = wave_uniform(sh->s.inv_dy2);
+            a.s.inv_dz2 = wave_uniform(sh->s.inv_dz2);
+            a.s.dt2 = wave_uniform(sh->s.dt2);
             if (a.mirror) a.src_z = a.nz - 1 - a.src_z;
-            item_tx = it.tile_x;
-            item_ty = it.tile_y;
-            item_ch = it.chunk_id;
+            item_tx = wave_uniform(it.tile_x);
+            item_ty = wave_uniform(it.tile_y);
+            item_ch = wave_uniform(it.chunk_id);
         } else if (a.batch) {  // uniform per workgroup
             // Fields read straight from the table (a private copy of the entry
             // indexed by rot would live in scratch memory).
@@ -293,7 +316,7 @@ void zq_kernel(ZqArgs a) {
 
     int tile_x = item_tx, tile_y = item_ty, chunk_id = item_ch;
     if (item_tx < 0) {
-        const unsigned logical = dev::xcd_remap(blockIdx.x, gridDim.x, a.xcd_remap != 0);
+        const unsigned logical = dev::xcd_remap(block, gridDim.x, a.xcd_remap != 0);
         tile_x = static_cast<int>(logical % static_cast<unsigned>(a.tiles_x));
         const unsigned rest = logical / static_cast<unsigned>(a.tiles_x);
         tile_y = static_cast<int>(rest % static_cast<unsigned>(a.tiles_y));
@@ -721,7 +744,109 @@ void zq_kernel(ZqArgs a) {
     }
 }
 
+template <int TXV, int TY, int MODE, bool NT, int FMT = dev::kF32>
+__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, FMT, MODE))))
+void zq_kernel(ZqArgs a) {
+    zq_tile<TXV, TY, MODE, NT, FMT>(a, blockIdx.x);
+}
+
+// The whole ragged batch run in one launch.  Work item j of XCD run r
+// (r = 0..7) is item 8 j + r of the interleaved item table, run_len =
+// ceil(n_items / 8) items per run.  In each step a workgroup of run r
+// (blockIdx.x % 8 == r, so it sits on that run's XCD) takes the next item of
+// its run from a queue, as the dispatcher of a plain launch would hand it
+// the next workgroup, until the run is empty; then the grid meets at a
+// barrier before the next step reads what this one wrote.
+//
+// The counters only grow: per step, run r's queue hands out run_len items
+// and one empty slot to each of its G / 8 workgroups, so step n's slots start
+// at n (run_len + G / 8); the barrier counter reaches (n + 1) G when every
+// workgroup has finished step n.  One lane per workgroup spins, sleeping
+// between polls, for at most kBarrierPolls polls (seconds; a step takes
+// milliseconds): the host launches no more workgroups than stay resident
+// together, and a barrier that still times out sets the abort flag, which
+// makes every workgroup leave at its next barrier, so the kernel always
+// ends.  The fences order this workgroup's stores of the step before its
+// arrival and the next step's loads after the barrier, at device scope
+// (across the XCDs' L2s).
+//
+// sync: [0] abort flag, [1] barrier counter, [2 + r] queue of run r; zero
+// first.
+constexpr unsigned kBarrierPolls = 1u << 22;
+
+__device__ bool grid_barrier(unsigned* sync, unsigned target) {
+    __shared__ int s_ok;
+    __syncthreads();  // every lane's stores of the step issued
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        for (unsigned polls = 0; __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+             ++polls) {
+            if (polls >= kBarrierPolls ||
+                ((polls & 63u) == 0 && __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <int TXV, int TY, bool NT>
+__global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, dev::kF32, kWaveFwi))))
+void zq_persist_kernel(const RaggedShot* shots, const RaggedItem* items, int n_steps, int n_items, int alternate,
+                       unsigned* sync) {
+    __shared__ int s_item;
+    const unsigned run = blockIdx.x % 8u;
+    const unsigned run_len = (static_cast<unsigned>(n_items) + 7u) / 8u;
+    const unsigned per_step = run_len + gridDim.x / 8u;  // queue slots of one step
+    unsigned* const queue = sync + 2 + run;
+    for (int n = 0; n < n_steps; ++n) {
+        ZqArgs s{};  // fields the item does not set stay known zeros (no trace, no L p output)
+        s.items = items;
+        s.rshots = shots;
+        s.rot = n % 3;
+        s.step = n;
+        s.mirror = alternate ? (n & 1) : 0;
+        const unsigned first = static_cast<unsigned>(n) * per_step;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                const unsigned k =
+                    __hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - first;
+                s_item = k < run_len ? static_cast<int>(k * 8u + run) : -1;
+            }
+            __syncthreads();
+            // one value for the workgroup: readfirstlane tells the compiler so
+            const int item = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();  // s_item read before lane 0 writes the next
+            if (item < 0) break;
+            if (item < n_items) zq_tile<TXV, TY, kWaveFwi, NT>(s, static_cast<unsigned>(item));
+        }
+        if (n + 1 < n_steps && !grid_barrier(sync, static_cast<unsigned>(n + 1) * gridDim.x)) return;
+    }
+}
+
 using KernelFn = void (*)(ZqArgs);
+using PersistFn = void (*)(const RaggedShot*, const RaggedItem*, int, int, int, unsigned*);
+
+template <bool NT>
+PersistFn pick_persist(int txv, int ty) {
+    if (txv == 16 && ty == 16) return zq_persist_kernel<16, 16, NT>;
+    if (txv == 16 && ty == 32) return zq_persist_kernel<16, 32, NT>;
+    if (txv == 32 && ty == 8) return zq_persist_kernel<32, 8, NT>;
+    if (txv == 32 && ty == 16) return zq_persist_kernel<32, 16, NT>;
+    if (txv == 64 && ty == 4) return zq_persist_kernel<64, 4, NT>;
+    if (txv == 64 && ty == 8) return zq_persist_kernel<64, 8, NT>;
+    if (txv == 64 && ty == 12) return zq_persist_kernel<64, 12, NT>;
+    if (txv == 64 && ty == 16) return zq_persist_kernel<64, 16, NT>;
+    return nullptr;
+}
 
 // Tile shapes (float4 columns x rows) the library is built with.
 struct TileShape {
@@ -1018,6 +1143,33 @@ hipError_t ragged_zq_fwi_f32(const RaggedShot* shots, const RaggedItem* items, i
     a.step = step;
     a.mirror = sweep_alternation() ? (step & 1) : 0;  // every shot chains step after step
     hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(n_items)), dim3(txv * ty), 0, st, a);
+    return hipGetLastError();
+}
+
+int ragged_persist_grid(int txv, int ty) {
+    PersistFn fn = pick_persist<true>(txv, ty);
+    if (!fn) return 0;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), txv * ty, 0) !=
+            hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return (per_cu * cus) / 8 * 8;
+}
+
+hipError_t ragged_persist_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int n_steps, int txv,
+                                  int ty, int grid, unsigned* sync, hipStream_t st) {
+    if (n_items <= 0 || n_steps <= 0) return hipSuccess;
+    PersistFn fn = pick_persist<true>(txv, ty);
+    // the queues ((run_len + grid / 8) slots per step) and the barrier
+    // counter (grid per step) stay below 2^32
+    const uint64_t bound = (static_cast<uint64_t>(n_items) + static_cast<uint64_t>(grid)) * static_cast<uint64_t>(n_steps);
+    if (!fn || !shots || !items || !sync || grid < 8 || grid % 8 != 0 || bound >= (uint64_t(1) << 32))
+        return hipErrorInvalidValue;
+    const int alternate = sweep_alternation() ? 1 : 0;
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(grid)), dim3(static_cast<unsigned>(txv * ty)), 0, st, shots,
+                       items, n_steps, n_items, alternate, sync);
     return hipGetLastError();
 }
 

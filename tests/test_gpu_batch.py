@@ -1,9 +1,9 @@
 """Batched multi-shot forward modelling on the GPU: every shot of a ragged
 batch equals the oracle's single-shot run bit for bit (default, tuned and every
 launch plan: each built tile shape with short, medium and whole-column
-z-chunks, with and without the per-XCD runs; and the grouped mode at every
-built shape), reruns of the captured graph are identical, and bad shots are
-rejected."""
+z-chunks, with and without the per-XCD runs, as one launch per step and as one
+persistent launch; and the grouped mode at every built shape), reruns of the
+captured graph are identical, and bad shots are rejected."""
 from __future__ import annotations
 
 import numpy as np
@@ -39,12 +39,12 @@ def _shots(ss, dev, n_steps, seed=0, with_source=True):
     return shots, host
 
 
-@pytest.fixture(params=["ragged", "grouped"])
+@pytest.fixture(params=["ragged", "persistent", "grouped"])
 def mode(request, monkeypatch):
-    if request.param == "grouped":
-        monkeypatch.setenv("STENCIL_BATCH_MODE", "grouped")
-    else:
+    if request.param == "ragged":
         monkeypatch.delenv("STENCIL_BATCH_MODE", raising=False)
+    else:
+        monkeypatch.setenv("STENCIL_BATCH_MODE", request.param)
     return request.param
 
 
@@ -52,7 +52,9 @@ def test_batch_matches_oracle_per_shot(ss, dev, oracle, mode):
     n = 30
     shots, host = _shots(ss, dev, n)
     b = ss.ShotBatch(shots, n, dev, tune=False)
-    assert b.ragged == (mode == "ragged")
+    assert b.ragged == (mode != "grouped")
+    assert (b.persistent_grid > 0) == (mode == "persistent")
+    assert b.persistent_grid % 8 == 0
     b.run()
     for i, (m, w, src) in enumerate(host):
         want = oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w)
@@ -91,17 +93,22 @@ def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev, mode):
 TILES = [(16, 16), (16, 32), (32, 8), (32, 16), (64, 4), (64, 8), (64, 12), (64, 16)]
 
 
-def test_ragged_batch_every_plan_matches_oracle(ss, dev, oracle, monkeypatch):
-    """The one-launch-per-step batch at every built tile shape, with z-chunks
-    of 9 planes (several chunks per column, ragged last chunk), 18 and whole
-    columns, with and without the per-XCD runs: every shot equals the
-    oracle's single-shot run bit for bit."""
-    monkeypatch.delenv("STENCIL_BATCH_MODE", raising=False)
+@pytest.mark.parametrize("persistent", [False, True])
+def test_ragged_batch_every_plan_matches_oracle(ss, dev, oracle, monkeypatch, persistent):
+    """The ragged batch -- one launch per step, or one persistent launch for
+    the run -- at every built tile shape, with z-chunks of 9 planes (several
+    chunks per column, ragged last chunk), 18 and whole columns, with and
+    without the per-XCD runs: every shot equals the oracle's single-shot run
+    bit for bit."""
+    if persistent:
+        monkeypatch.setenv("STENCIL_BATCH_MODE", "persistent")
+    else:
+        monkeypatch.delenv("STENCIL_BATCH_MODE", raising=False)
     n = 12
     shots, host = _shots(ss, dev, n, seed=5)
     want = [oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w) for m, w, src in host]
     b = ss.ShotBatch(shots, n, dev, tune=False)
-    assert b.ragged
+    assert b.ragged and (b.persistent_grid > 0) == persistent
     for tx4, ty in TILES:
         for chunk in (9, 18, 1000):
             for runs in (True, False):
