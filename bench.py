@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU-baseline sample length per row (two rows: no FTZ/DAZ, FTZ/DAZ)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default shape)")
+    ap.add_argument("--shape", default=None, metavar="TX,TY,CHUNKS,XCD",
+                    help="FP32 tiled step at this launch shape (tile_x cells, tile_y rows, z-chunks, XCD "
+                         "remap 0/1) instead of the tuned one; implies --no-tune for that step")
     ap.add_argument("--no-precision", action="store_true", help="skip the 16-bit storage / FP64 study")
     ap.add_argument("--precision-n", type=int, nargs="*", default=None,
                     help="grid sizes of the 16-bit storage study (default: --n, and 1024 when --n is 512)")
@@ -144,7 +147,14 @@ def main():
         return ms, out
 
     tuned = None
-    if args.variant == "tiled" and not args.no_tune:
+    if args.shape:
+        if args.variant != "tiled" or args.storage != "fp32":
+            sys.exit("--shape applies to the FP32 tiled step")
+        tx, ty, nc, xr = (int(v) for v in args.shape.split(","))
+        ss.set_tuned_shape(grid, "wave", {"tile_x": tx, "tile_y": ty, "n_chunks": nc, "xcd_remap": xr,
+                                          "nontemporal": 1})
+        print(f"[bench] fixed shape {ss.tuned_shape(grid, 'wave')[0]}", file=sys.stderr, flush=True)
+    elif args.variant == "tiled" and not args.no_tune:
         # One-time autotune of the launch shape for this grid size (outside the
         # timed region; the launcher caches the result).
         tb = [p0.clone(), p0.clone(), torch.zeros_like(p0)]  # distinct buffers, as in the run
