@@ -152,7 +152,9 @@ int stencil_tune_heat_f32(const float* t, float* t_next, const stencil_grid* gri
                           stencil_tile_shape* best, const char* csv_path, void* hip_stream);
 /* The shape a NULL-shape launch would use for these dims (kind 0 wave, 1
  * heat, 2 + storage for the 16-bit wave step, 6 FWI forward step, 7 adjoint
- * step, 8 forward step recomputed beside the adjoint): returns 1 if it came
+ * step, 8 forward step recomputed beside the adjoint, 9 the backward pass's
+ * step pair: a recompute and an adjoint step in one launch, built for tiles
+ * of 128 x 8, 128 x 16, 256 x 4 and 256 x 8 cells): returns 1 if it came
  * from the tuner cache, 0 for the default. */
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out);
 /* Puts a shape into the tuner cache for these dims and kind (as above; kind 2

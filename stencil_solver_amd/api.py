@@ -202,13 +202,14 @@ def tune_heat(t, t_next, grid: GridDims, reps: int = 5, csv_path=None, stream=No
 
 # Tuner-cache kinds (the C-ABI's kind numbers).
 _SHAPE_KINDS = {"wave": 0, "heat": 1, "bf16": 3, "fp16": 4, "bfp16": 5, "fwi_forward": 6, "fwi_adjoint": 7,
-                "fwi_recompute": 8}
+                "fwi_recompute": 8, "fwi_pair": 9}
 
 
 def tuned_shape(grid: GridDims, kind: str = "wave"):
     """(shape dict, from_tuner) a NULL-shape launch would use on these dims;
     kind "wave", "heat", a 16-bit storage name for launch_tiled_16, or
-    "fwi_forward" / "fwi_adjoint" / "fwi_recompute" for the FWI steps."""
+    "fwi_forward" / "fwi_adjoint" / "fwi_recompute" / "fwi_pair" (the
+    backward pass's recompute + adjoint step pair) for the FWI steps."""
     shp = stencil_tile_shape()
     g = grid.to_c()
     r = lib().stencil_tuned_shape(C.byref(g), _SHAPE_KINDS[kind], C.byref(shp))

@@ -52,6 +52,9 @@ bool zq_shape_valid(int txv, int ty);
 constexpr int kKindFwiForward = 6;
 constexpr int kKindFwiAdjoint = 7;
 constexpr int kKindFwiRecompute = 8;  // forward steps beside the adjoint steps
+constexpr int kKindFwiPair = 9;       // the backward pass's step pair (recompute + adjoint, one launch)
+// Whether (txv, ty) is a tile shape the step pair is built with.
+bool zq_pair_shape_valid(int txv, int ty);
 // Fills unset fields of a shape for a kernel of this kind; the adjoint kind
 // also maps the tiles it is not built for (64 x 4, 64 x 16) to 64 x 8.
 ZqShape zq_resolve(const Dims& d, ZqShape s, int kind);
@@ -108,6 +111,15 @@ hipError_t wave_zq_fwi_f32(const float* p, const float* pp, const float* m, floa
 hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
                           float* grad, float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
                           float* trace, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st);
+// The backward pass's step pair: one launch runs the adjoint step above
+// (psi1 .. trace) and the forward step p, pp -> pn with L p into lap_out and
+// *src_val at the source cell (no receiver plane) on the same tiles, reading
+// m once; each field gets the bits the two single steps would give it.  The
+// two steps' fields must be distinct buffers.
+hipError_t pair_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
+                       float* grad, float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+                       float* trace, const float* p, const float* pp, float* pn, float* lap_out,
+                       const float* src_val, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st);
 // One shot of a batched forward launch (device table entry): three time-level
 // buffers, vel2, the source cell and its wavelet (nullable: no source).
 struct BatchShot {

@@ -245,7 +245,7 @@ int stencil_tune_tiled_16(int storage, const stencil_field16* p_cur, const stenc
 }
 
 int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* out) {
-    if (!dims_ok(grid) || !out || kind < 0 || kind > kKindFwiRecompute) return STENCIL_ERR_INVALID;
+    if (!dims_ok(grid) || !out || kind < 0 || kind > kKindFwiPair) return STENCIL_ERR_INVALID;
     Dims d;
     d.nx = int64_t(grid->nx);
     d.ny = int64_t(grid->ny);
@@ -262,8 +262,10 @@ int stencil_tuned_shape(const stencil_grid* grid, int kind, stencil_tile_shape* 
 }
 
 int stencil_set_tuned_shape(const stencil_grid* grid, int kind, const stencil_tile_shape* shape) {
-    if (!dims_ok(grid) || !shape || kind < 0 || kind > kKindFwiRecompute || kind == 2) return STENCIL_ERR_INVALID;
-    if (!zq_shape_valid(shape->tile_x4, shape->tile_y) || shape->n_chunks < 0 || shape->xcd_remap < -1 ||
+    if (!dims_ok(grid) || !shape || kind < 0 || kind > kKindFwiPair || kind == 2) return STENCIL_ERR_INVALID;
+    const bool built = kind == kKindFwiPair ? zq_pair_shape_valid(shape->tile_x4, shape->tile_y)
+                                            : zq_shape_valid(shape->tile_x4, shape->tile_y);
+    if (!built || shape->n_chunks < 0 || shape->xcd_remap < -1 ||
         shape->xcd_remap > 1 || shape->nontemporal < -1 || shape->nontemporal > 1)
         return STENCIL_ERR_INVALID;
     Dims d;

@@ -48,7 +48,19 @@ using dev::kR;
 // output (FWI forward / recompute); kAdjoint: the adjoint step in psi = m lambda,
 // psi_out = 2 psi1 - psi2 + dt^2 m L psi1 with m times the residual on the
 // receiver plane and the gradient update (FWI backward).
-enum Mode : int { kWaveFused = 0, kWaveExact = 1, kHeatFused = 2, kHeatExact = 3, kWaveFwi = 4, kAdjoint = 5 };
+// kPair: one launch runs an adjoint step (the kAdjoint channel: cur, prev,
+// next, aux, aux2, grad, res, trace) and a recompute step (the kWaveFwi
+// channel: cur_b, prev_b, next_b, lap_out, src_val; no receiver plane) of the
+// FWI backward pass on the same tiles, reading m (vel2) once for both.
+enum Mode : int {
+    kWaveFused = 0,
+    kWaveExact = 1,
+    kHeatFused = 2,
+    kHeatExact = 3,
+    kWaveFwi = 4,
+    kAdjoint = 5,
+    kPair = 6
+};
 
 struct ZqArgs {
     const void* cur;
@@ -103,6 +115,11 @@ struct ZqArgs {
     // alternating them between chained steps starts each step on the planes
     // the previous one touched last, which are still in the memory-side cache.
     int mirror;
+    // kPair's recompute channel: p^j, p^{j-1}, p^{j+1} (lap_out = L p^j,
+    // src_val = &wavelet[j]).
+    const void* cur_b;
+    const void* prev_b;
+    void* next_b;
 };
 
 using dev::Fmt;
@@ -219,6 +236,9 @@ constexpr int prefetch_depth(int txv, int ty, int mode) {
 // planes in flight and two halo items, and the adjoint.
 constexpr int min_waves(int txv, int ty, int fmt, int mode) {
     const int threads = txv * ty;
+    // the step pair holds two register queues and two stencils' streams:
+    // 256 VGPRs, one 512-thread or two 256-thread workgroups per CU
+    if (mode == kPair) return 2;
     const bool shuffle_x = txv == 64 && fmt == dev::kF32;  // x neighbours by lane shifts: no column items
     const int halo_per = (2 * dev::kR * txv + (shuffle_x ? 0 : 2 * ty) + threads - 1) / threads;
     if (threads == 768 || halo_per >= 3 || (halo_per == 2 && (fmt == dev::kBfp16 || threads == 256))) return 3;
@@ -251,8 +271,12 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     constexpr bool kHeat = (MODE == kHeatFused || MODE == kHeatExact);
     constexpr bool kFwd = (MODE == kWaveFwi);
     constexpr bool kAdj = (MODE == kAdjoint);
-    constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact || kFwd || kAdj);
-    static_assert(!(kFwd || kAdj) || FMT == dev::kF32, "FWI modes are FP32");
+    constexpr bool kPairM = (MODE == kPair);
+    constexpr bool kExact = (MODE == kWaveExact || MODE == kHeatExact || kFwd || kAdj || kPairM);
+    static_assert(!(kFwd || kAdj || kPairM) || FMT == dev::kF32, "FWI modes are FP32");
+    // channels: the step pair stages two fields per plane (LDS set 1 is the
+    // recompute's p, set 0 the adjoint's psi); every other mode one
+    constexpr int kNch = kPairM ? 2 : 1;
     int item_tx = -1, item_ty = 0, item_ch = 0;  // ragged work item (tile, chunk)
     if constexpr (kFwd) {
         if (a.items) {  // uniform per workgroup
@@ -312,7 +336,7 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     // left/right columns unless the x neighbours come by lane shifts
     constexpr int kHalo = 2 * kR * TXV + (kShuffleX ? 0 : 2 * TY);
     constexpr int kHaloPer = (kHalo + kThreads - 1) / kThreads;
-    __shared__ float4 lds[2][kRows][kCols];
+    __shared__ float4 lds[kNch][2][kRows][kCols];
 
     int tile_x = item_tx, tile_y = item_ty, chunk_id = item_ch;
     if (item_tx < 0) {
@@ -442,6 +466,9 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     const Rsrc aux2_r = field(a.aux2, plane_bytes);
     const Rsrc grad_r = field(a.grad, plane_bytes);
     const Rsrc lap_r = field(a.lap_out, plane_bytes);
+    const Rsrc cur_b_r = field(a.cur_b, plane_bytes);
+    const Rsrc prev_b_r = field(a.prev_b, plane_bytes);
+    const Rsrc next_b_r = field(a.next_b, plane_bytes);
     const Rsrc cur_er = field(a.cur_e, eplane);
     const Rsrc prev_er = field(a.prev_e, eplane);
     const Rsrc vel_er = field(a.vel2_e, eplane);
@@ -457,10 +484,14 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     // the raw mantissas of a block-FP16 vel2 block).
     // lc: L p^k at the point (kAdjoint's gradient term); lap4 receives L p
     // (kWaveFwi).
+    // The helpers below take the mode of the channel they serve as a
+    // compile-time argument (cm: std::integral_constant): MODE itself, or for
+    // the step pair kAdjoint (LDS set 0) and kWaveFwi (LDS set 1).
     // The x neighbour float4s of this lane's centre (plane in LDS buffer b).
     // Called by every lane, outside any divergent branch: the lane shifts
     // read the adjacent lanes' registers, which must be active.
-    auto x_neighbours = [&](int b, float4 ctr4, float4 edge, float4& xl, float4& xr) {
+    auto x_neighbours = [&](auto cm, int b, float4 ctr4, float4 edge, float4& xl, float4& xr) {
+        constexpr int ch = (kPairM && decltype(cm)::value == kWaveFwi) ? 1 : 0;
         if constexpr (kShuffleX) {
             // from the adjacent lanes' centre float4 (a wave is one tile row);
             // the row's end lanes take their edge float4 instead.
@@ -469,12 +500,14 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
             xr = dev::wave_shl1(ctr4, edge);
         } else {
             (void)edge;
-            xl = lds[b][ty + kR][tx];
-            xr = lds[b][ty + kR][tx + 2];
+            xl = lds[ch][b][ty + kR][tx];
+            xr = lds[ch][b][ty + kR][tx + 2];
         }
     };
-    auto compute = [&](int b, const float4 (&zq)[9], float4 xl, float4 xr, float4 pv, float4 vv, float dt2v,
-                       float4 lc, float4& lap4) -> float4 {
+    auto compute = [&](auto cm, int b, const float4 (&zq)[9], float4 xl, float4 xr, float4 pv, float4 vv,
+                       float dt2v, float4 lc, float4& lap4) -> float4 {
+        constexpr int ch = (kPairM && decltype(cm)::value == kWaveFwi) ? 1 : 0;
+        constexpr bool cFwd = decltype(cm)::value == kWaveFwi;
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ctr4 = zq[kR];
         const float w[12] = {xl.x,   xl.y,   xl.z,   xl.w,   ctr4.x, ctr4.y,
@@ -482,8 +515,8 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
         float4 ym[kR], yp[kR];
 #pragma unroll
         for (int r = 1; r <= kR; ++r) {
-            ym[r - 1] = lds[b][ty + kR - r][tx + 1];
-            yp[r - 1] = lds[b][ty + kR + r][tx + 1];
+            ym[r - 1] = lds[ch][b][ty + kR - r][tx + 1];
+            yp[r - 1] = lds[ch][b][ty + kR + r][tx + 1];
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -504,7 +537,7 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                     res = p + a.coef * (lx + ly + lz);
                 } else {  // wave, FWI forward, adjoint in psi (vv = m)
                     res = 2.f * p - comp(pv, i) + a.s.dt2 * comp(vv, i) * (lx + ly + lz);
-                    if constexpr (kFwd) dev::set_comp(lap4, i, lx + ly + lz);
+                    if constexpr (cFwd) dev::set_comp(lap4, i, lx + ly + lz);
                 }
             } else {
                 // Combined weights w_r = c_r / h^2, one accumulator.
@@ -528,25 +561,30 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
     };
     // Steps past z_end (the tail of the last group of 9 in a chunk whose length
     // is not a multiple of 9) run on clamped planes and store nothing.
-    auto finish = [&](int zz, int b, const float4 (&zq)[9], float4 edge, float4 pv, float4 vv, float dt2v,
-                      float4 lc, float4 gr, float4 l2) {
+    auto finish = [&](auto cm, int zz, int b, const float4 (&zq)[9], float4 edge, float4 pv, float4 vv,
+                      float dt2v, float4 lc, float4 gr, float4 l2) {
+        constexpr bool cFwd = decltype(cm)::value == kWaveFwi;
+        constexpr bool cAdj = decltype(cm)::value == kAdjoint;
         // Every lane computes (its inputs are loaded, from offset 0 outside
         // the grid); only output lanes store, and block-FP16 leaves the other
         // lanes out of its block maximum.
         float4 lap4 = make_float4(0.f, 0.f, 0.f, 0.f);
         float4 xl, xr;
-        x_neighbours(b, zq[kR], edge, xl, xr);
-        float4 out = compute(b, zq, xl, xr, pv, vv, dt2v, lc, lap4);
+        x_neighbours(cm, b, zq[kR], edge, xl, xr);
+        float4 out = compute(cm, b, zq, xl, xr, pv, vv, dt2v, lc, lap4);
         if (zz < z_end) {
-            if constexpr (kFwd || kAdj) {
+            if constexpr (cFwd || cAdj) {
                 // the source / trace cell lies in this lane's 4 cells of this plane
                 const bool at_src = own_out && zz == a.src_z && y == a.src_y && a.src_x >= x && a.src_x < x + 4;
-                if constexpr (kFwd) {
+                if constexpr (cFwd) {
                     if (at_src && a.src_val) dev::set_comp(out, a.src_x - x, comp(out, a.src_x - x) + *a.src_val);
-                    // receiver plane: p^{n+1} on plane rec_z into the trace (its halo is pre-zeroed)
-                    if (a.trace && zz == a.rec_z)
-                        store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), 0, plane_rsrc(a.trace, 0, 0),
-                                                 0, own_b, 0, out, 1.f, own_out, false);
+                    // receiver plane: p^{n+1} on plane rec_z into the trace (its
+                    // halo is pre-zeroed); the recompute of a step pair records none
+                    if constexpr (!kPairM)
+                        if (a.trace && zz == a.rec_z)
+                            store4<FMT, false, true>(plane_rsrc(a.trace, 0, plane_bytes), 0,
+                                                     plane_rsrc(a.trace, 0, 0), 0, own_b, 0, out, 1.f, own_out,
+                                                     false);
                     if (a.lap_out)  // written once, read much later: non-temporal in either mode
                         store4<FMT, true, true>(lap_r, soff(zz, plane_bytes), lap_r, 0, own_b, 0, lap4, 1.f,
                                                 own_out, false);
@@ -570,10 +608,15 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                     if (at_src && a.trace) a.trace[0] = comp(out, a.src_x - x) / comp(vv, a.src_x - x);
                 }
             }
-            store4<FMT, NT, kExact>(next_r, soff(zz, plane_bytes), next_er, kBlk ? soff(zz, eplane) : 0, own_b,
-                                    own_off >> 4, out, a.p_scale, own_out, blk_out);
+            if constexpr (kPairM && cFwd)
+                store4<FMT, NT, true>(next_b_r, soff(zz, plane_bytes), next_b_r, 0, own_b, 0, out, 1.f, own_out,
+                                      false);
+            else
+                store4<FMT, NT, kExact>(next_r, soff(zz, plane_bytes), next_er, kBlk ? soff(zz, eplane) : 0, own_b,
+                                        own_off >> 4, out, a.p_scale, own_out, blk_out);
         }
     };
+    using ModeC = std::integral_constant<int, MODE>;
 
     // Register queue: plane (zz - 4 + k) lives in q[(j + k) % 9] at unroll step j.
     float4 q[9];
@@ -587,7 +630,79 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                                        exp_at(cur_er, z_begin - kR + k, own_eld), p_inv);
     }
 
-    if constexpr (FMT == dev::kF32) {
+    if constexpr (kPairM) {
+        // The step pair: the adjoint channel (queue q, LDS set 0) and the
+        // recompute channel (queue qb, LDS set 1) march the same planes, one
+        // barrier per plane for both; m is loaded once and serves both steps.
+        // Every load goes out one plane ahead (queue refills: four planes
+        // further), as in the single FWI steps.
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool lap_in = a.aux != nullptr;
+        const bool lap2_in = lap_in && a.aux2 != nullptr;
+        float4 qb[9];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) qb[k] = raw_at(cur_b_r, z_begin - kR + k, own_ld);
+        struct Stage {
+            float4 r, rb, xe, xeb, h[kHaloPer], hb[kHaloPer], pv, pvb, vv, lc, gr, l2;
+        };
+        // loads of plane z (queue refills: plane z + kR)
+        auto fetch = [&](int z, Stage& st) {
+            st.r = raw_at(cur_r, min(z + kR, z_last), own_ld);
+            st.rb = raw_at(cur_b_r, min(z + kR, z_last), own_ld);
+            st.xe = st.xeb = zero4;
+            if constexpr (kShuffleX) {
+                st.xe = raw_at(cur_r, z, xe_ld);
+                st.xeb = raw_at(cur_b_r, z, xe_ld);
+            }
+#pragma unroll
+            for (int k = 0; k < kHaloPer; ++k) {
+                st.h[k] = raw_at(cur_r, z, h_ld[k]);
+                st.hb[k] = raw_at(cur_b_r, z, h_ld[k]);
+            }
+            st.pv = raw_stream(prev_r, z);
+            st.pvb = raw_stream(prev_b_r, z);
+            st.vv = raw_stream(vel_r, z);
+            st.lc = st.gr = st.l2 = zero4;
+            if (lap_in) {
+                st.lc = raw_once(aux_r, z);
+                st.gr = raw_stream(grad_r, z);
+            }
+            if (lap2_in) st.l2 = raw_once(aux2_r, z);
+        };
+        Stage cs;
+        fetch(min(z_begin, z_last), cs);
+        for (int zb = z_begin; zb < z_end; zb += 9) {
+            dev::static_for<9>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int zz = zb + j;
+                q[(j + 8) % 9] = cs.r;
+                qb[(j + 8) % 9] = cs.rb;
+                Stage ns;
+                fetch(min(zz + 1, z_last), ns);
+                const int b = (zz - z_begin) & 1;
+                lds[0][b][ty + kR][tx + 1] = q[(j + 4) % 9];
+                lds[kNch - 1][b][ty + kR][tx + 1] = qb[(j + 4) % 9];
+#pragma unroll
+                for (int k = 0; k < kHaloPer; ++k)
+                    if (tid + k * kThreads < kHalo) {
+                        lds[0][b][h_row[k]][h_col[k]] = cs.h[k];
+                        lds[kNch - 1][b][h_row[k]][h_col[k]] = cs.hb[k];
+                    }
+                __syncthreads();
+                const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
+                                      q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
+                                      q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
+                const float4 zqb[9] = {qb[j % 9],       qb[(j + 1) % 9], qb[(j + 2) % 9],
+                                       qb[(j + 3) % 9], qb[(j + 4) % 9], qb[(j + 5) % 9],
+                                       qb[(j + 6) % 9], qb[(j + 7) % 9], qb[(j + 8) % 9]};
+                finish(std::integral_constant<int, kAdjoint>{}, zz, b, zq, cs.xe, cs.pv, cs.vv, a.s.dt2, cs.lc,
+                       cs.gr, cs.l2);
+                finish(std::integral_constant<int, kWaveFwi>{}, zz, b, zqb, cs.xeb, cs.pvb, cs.vv, a.s.dt2, zero4,
+                       zero4, zero4);
+                cs = ns;
+            });
+        }
+    } else if constexpr (FMT == dev::kF32) {
         // FP32: every load is issued kPD steps ahead: the queue refill for
         // plane zz+4+kPD, halo and streams for plane zz+kPD; stage d of each
         // FIFO holds plane zz+d (queue refills: zz+4+d).  (kAdjoint streams
@@ -633,15 +748,15 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                 float4 r_n, xe_n, h_n[kHaloPer], pv_n, vv_n, lc_n, gr_n, l2_n;
                 fetch(min(zz + kPD, z_last), r_n, xe_n, h_n, pv_n, vv_n, lc_n, gr_n, l2_n);
                 const int b = (zz - z_begin) & 1;
-                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
+                lds[0][b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
-                    if (tid + k * kThreads < kHalo) lds[b][h_row[k]][h_col[k]] = hq[0][k];
+                    if (tid + k * kThreads < kHalo) lds[0][b][h_row[k]][h_col[k]] = hq[0][k];
                 __syncthreads();
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, xeq[0], pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0], l2q[0]);
+                finish(ModeC{}, zz, b, zq, xeq[0], pvq[0], vvq[0], a.s.dt2, lcq[0], grq[0], l2q[0]);
                 // FIFO shift (register renaming under the unroll)
 #pragma unroll
                 for (int d = 0; d + 1 < kPD; ++d) {
@@ -712,10 +827,10 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                     rh_e[k] = exp_at(cur_er, z1, h_eld[k]);
                 }
                 const int b = (zz - z_begin) & 1;
-                lds[b][ty + kR][tx + 1] = q[(j + 4) % 9];
+                lds[0][b][ty + kR][tx + 1] = q[(j + 4) % 9];
 #pragma unroll
                 for (int k = 0; k < kHaloPer; ++k)
-                    if (h_use[k]) lds[b][h_row[k]][h_col[k]] = h[k];
+                    if (h_use[k]) lds[0][b][h_row[k]][h_col[k]] = h[k];
                 __syncthreads();
                 // after it: the queue's newest plane and the streams of plane zz
                 q[(j + 8) % 9] = decode<FMT, kExact>(rq, rq_e, p_inv);
@@ -738,7 +853,7 @@ __device__ __forceinline__ void zq_tile(ZqArgs a, unsigned block) {
                 const float4 zq[9] = {q[j % 9],       q[(j + 1) % 9], q[(j + 2) % 9],
                                       q[(j + 3) % 9], q[(j + 4) % 9], q[(j + 5) % 9],
                                       q[(j + 6) % 9], q[(j + 7) % 9], q[(j + 8) % 9]};
-                finish(zz, b, zq, pv, pv, vv, dt2v, pv, pv, pv);
+                finish(ModeC{}, zz, b, zq, pv, pv, vv, dt2v, pv, pv, pv);
             }
         }
     }
@@ -869,8 +984,22 @@ KernelFn pick_shape(int txv, int ty) {
     return nullptr;
 }
 
+// The step pair's tiles: two queues and two LDS planes per workgroup fit 256
+// VGPRs and 68 KB of LDS at these shapes.
+constexpr TileShape kPairShapes[] = {{32, 8}, {32, 16}, {64, 4}, {64, 8}};
+
+template <bool NT>
+KernelFn pick_pair(int txv, int ty) {
+    if (txv == 32 && ty == 8) return zq_kernel<32, 8, kPair, NT>;
+    if (txv == 32 && ty == 16) return zq_kernel<32, 16, kPair, NT>;
+    if (txv == 64 && ty == 4) return zq_kernel<64, 4, kPair, NT>;
+    if (txv == 64 && ty == 8) return zq_kernel<64, 8, kPair, NT>;
+    return nullptr;
+}
+
 KernelFn pick(int txv, int ty, int mode, bool nt) {
     switch (mode) {
+        case kPair: return nt ? pick_pair<true>(txv, ty) : pick_pair<false>(txv, ty);
         case kWaveFwi: return nt ? pick_shape<kWaveFwi, true>(txv, ty) : pick_shape<kWaveFwi, false>(txv, ty);
         case kAdjoint: return nt ? pick_shape<kAdjoint, true>(txv, ty) : pick_shape<kAdjoint, false>(txv, ty);
         case kWaveFused: return nt ? pick_shape<kWaveFused, true>(txv, ty) : pick_shape<kWaveFused, false>(txv, ty);
@@ -1016,6 +1145,12 @@ bool zq_shape_valid(int txv, int ty) {
     return false;
 }
 
+bool zq_pair_shape_valid(int txv, int ty) {
+    for (const auto& s : kPairShapes)
+        if (s.txv == txv && s.ty == ty) return true;
+    return false;
+}
+
 ZqShape zq_resolve(const Dims& d, ZqShape s, int kind) {
     if (!zq_shape_valid(s.txv, s.ty)) {
         // Widest tile whose row padding past nx is smallest (ties go wide):
@@ -1039,6 +1174,13 @@ ZqShape zq_resolve(const Dims& d, ZqShape s, int kind) {
     // thread) spill with the adjoint's extra streams: it runs 64 x 8 instead.
     if (kind == kKindFwiAdjoint && s.txv == 64 && (s.ty == 16 || s.ty == 4)) {
         s.ty = 8;
+        s.n_chunks = 0;
+    }
+    // The step pair is built for four tiles: 64-wide tiles keep 64 x 4 or
+    // become 64 x 8, narrower ones 32 x 8 or 32 x 16.
+    if (kind == kKindFwiPair && !zq_pair_shape_valid(s.txv, s.ty)) {
+        s.ty = s.txv == 64 ? 8 : (s.txv == 32 && s.ty == 8 ? 8 : 16);
+        s.txv = s.txv == 64 ? 64 : 32;
         s.n_chunks = 0;
     }
     if (s.nt < 0) s.nt = 1;
@@ -1194,6 +1336,35 @@ hipError_t adjoint_zq_f32(const float* psi1, const float* psi2, const float* m, 
     a.s = s;
     if (shape.nt < 0) shape.nt = 1;
     return launch_zq(a, d, zq_resolve(d, shape, kKindFwiAdjoint), kAdjoint, st);
+}
+
+hipError_t pair_zq_f32(const float* psi1, const float* psi2, const float* m, const float* lap, const float* lap2,
+                       float* grad, float* psi_out, const float* res, int rec_z, int src_z, int src_y, int src_x,
+                       float* trace, const float* p, const float* pp, float* pn, float* lap_out,
+                       const float* src_val, const Dims& d, const Scalars<float>& s, ZqShape shape, hipStream_t st) {
+    if ((lap2 && !lap) || !p || !pp || !pn) return hipErrorInvalidValue;
+    ZqArgs a{};
+    a.cur = psi1;
+    a.prev = psi2;
+    a.vel2 = m;
+    a.aux = lap;
+    a.aux2 = lap2;
+    a.grad = grad;
+    a.next = psi_out;
+    a.res = res;
+    a.rec_z = rec_z;
+    a.trace = trace;
+    a.cur_b = p;
+    a.prev_b = pp;
+    a.next_b = pn;
+    a.lap_out = lap_out;
+    a.src_val = src_val;
+    a.src_z = src_z;
+    a.src_y = src_y;
+    a.src_x = src_x;
+    a.s = s;
+    if (shape.nt < 0) shape.nt = 1;
+    return launch_zq(a, d, zq_resolve(d, shape, kKindFwiPair), kPair, st);
 }
 
 hipError_t heat_zq_f32(const float* t, float* tn, const Dims& d, const Scalars<float>& s,

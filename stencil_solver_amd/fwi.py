@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from ._lib import check, lib, stencil_fwi_config, stencil_tile_shape
-from .api import GridDims, _dev_ptr, _shape_dict, _stream, _variant
+from .api import GridDims, _dev_ptr, _shape_dict, _stream, _variant, tuned_shape
 
 
 def ricker(n_steps: int, dt: float, f_peak: float, t0: float | None = None, amplitude: float = 1.0) -> np.ndarray:
@@ -91,16 +91,21 @@ class FwiProblem:
     def tune(self, reps: int = 3, csv_path=None) -> dict:
         """Autotune the FP32 FWI kernels for this grid on the problem's own
         workspace (a forward pass's checkpoints, traces and stored L p stay
-        intact); returns {"forward", "adjoint", "recompute"} shapes: the
-        forward pass's steps, the adjoint steps, and the forward steps the
-        backward pass recomputes beside them."""
+        intact); returns {"forward", "adjoint", "recompute", "pair"} shapes:
+        the forward pass's steps, the adjoint steps, the forward steps the
+        backward pass recomputes beside them on a second stream, and the step
+        pair (a recompute and an adjoint step in one launch; None with a
+        single segment, where nothing is recomputed)."""
         if self.dtype != self.torch.float32 or self.variant != "tiled":
             raise ValueError("tune() applies to the FP32 tiled FWI kernels")
         shp = [stencil_tile_shape() for _ in range(3)]
         path = str(csv_path).encode() if csv_path else None
         check(lib().stencil_tune_fwi_f32(C.byref(self._g), C.byref(self.cfg), self.work.data_ptr(), int(reps),
                                          *[C.byref(x) for x in shp], path, _stream(None)), "tune_fwi")
-        return dict(zip(("forward", "adjoint", "recompute"), (_shape_dict(x) for x in shp)))
+        out = dict(zip(("forward", "adjoint", "recompute"), (_shape_dict(x) for x in shp)))
+        pair, hit = tuned_shape(self.grid, "fwi_pair")
+        out["pair"] = pair if hit else None
+        return out
 
     @property
     def ckpt_every(self) -> int:

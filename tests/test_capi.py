@@ -171,8 +171,17 @@ def test_set_tuned_shape_round_trip_and_validation():
         ss.set_tuned_shape(g, "wave", dict(shp, tile_y=7))  # not a built tile
     with pytest.raises(StencilError):
         ss.set_tuned_shape(g, "wave", dict(shp, xcd_remap=2))
+    # the step pair is built for four tiles only (128 x 8 / 16, 256 x 4 / 8 cells)
+    pair = dict(shp, tile_x=256, tile_y=4)
+    ss.set_tuned_shape(g, "fwi_pair", pair)
+    assert ss.tuned_shape(g, "fwi_pair") == (pair, True)
+    with pytest.raises(StencilError):
+        ss.set_tuned_shape(g, "fwi_pair", dict(shp, tile_x=256, tile_y=16))
     ss.clear_tuning()
     assert not ss.tuned_shape(g, "fwi_adjoint")[1]
+    # untuned, the pair maps the default shape onto a built one
+    got, hit = ss.tuned_shape(g, "fwi_pair")
+    assert not hit and (got["tile_x"], got["tile_y"]) in {(128, 8), (128, 16), (256, 4), (256, 8)}
 
 
 def test_hip_errors_are_reported_not_raised():

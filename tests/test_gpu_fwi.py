@@ -209,6 +209,73 @@ def test_fwi_every_tile_shape_equals_naive(ss, dev, shape):
     ss.clear_tuning()
 
 
+def _naive_reference(ss, dev, shape, n, ckpt, seed):
+    """(grid, m on the device, d_obs, J, gradient, source trace) of the
+    global-memory kernels on a smooth model, data from a 5 % faster model."""
+    import torch
+
+    src, rec_z = (shape[0] // 2, shape[1] // 2, shape[2] // 2), 5
+    m = smooth_model(shape, seed, 1800.0, 3000.0, np.float32)
+    w = (1e3 * ss.ricker(n, 1e-3, 20.0)).astype(np.float32)
+    grid = _grid(ss, shape, 10.0, 1e-3, np.float32)
+    md = torch.from_numpy(m).to(dev)
+    pn = ss.FwiProblem(grid, n, src, rec_z, w, dev, torch.float32, ckpt, "naive")
+    d_obs = pn.traces(md * 1.05)
+    J = pn.misfit(md, d_obs).item()
+    g, st = (t.cpu().numpy() for t in pn.gradient(md, src_trace=True))
+    assert np.abs(g).max() > 0
+    return grid, md, d_obs, J, g, st, (src, rec_z, w)
+
+
+@pytest.mark.parametrize("schedule", ["pair", "overlap", "serial"])
+@pytest.mark.parametrize("ckpt", [7, 1])
+def test_backward_schedules_equal_naive(ss, dev, monkeypatch, schedule, ckpt):
+    """The tiled FP32 backward pass in each schedule (STENCIL_FWI_SCHEDULE):
+    the step pair (a recompute step and an adjoint step in one launch), the
+    recompute on a second stream (the default), and one stream in turn.  All
+    give the global-memory kernels' misfit, gradient and source trace bit for
+    bit.  40 steps in segments of 7 (the short last segment leaves the first
+    paired round with single recompute steps) or of 1."""
+    import torch
+
+    shape, n = (36, 40, 104), 40
+    grid, md, d_obs, J_n, g_n, st_n, (src, rec_z, w) = _naive_reference(ss, dev, shape, n, ckpt, 5)
+    monkeypatch.setenv("STENCIL_FWI_SCHEDULE", schedule)
+    ss.clear_tuning()
+    pt = ss.FwiProblem(grid, n, src, rec_z, w, dev, torch.float32, ckpt, "tiled")
+    assert pt.misfit(md, d_obs).item() == J_n
+    g, st = pt.gradient(md, src_trace=True)
+    assert np.array_equal(g.cpu().numpy(), g_n)
+    assert np.array_equal(st.cpu().numpy(), st_n)
+
+
+@pytest.mark.parametrize("shape", [(30, 44, 72), (41, 20, 36)])
+def test_fwi_step_pair_every_shape_equals_naive(ss, dev, monkeypatch, shape):
+    """The step pair at each tile it is built for (128 x 8, 128 x 16, 256 x 4,
+    256 x 8 cells), 1 to 3 z-chunks, both workgroup orders and both streaming
+    modes, with the single recompute / adjoint steps at other shapes: misfit,
+    gradient and source trace bit-identical to the global-memory kernels on
+    ragged grids (four segments of 6 steps)."""
+    import torch
+
+    n = 24
+    grid, md, d_obs, J_n, g_n, st_n, (src, rec_z, w) = _naive_reference(ss, dev, shape, n, 6, 11)
+    monkeypatch.setenv("STENCIL_FWI_SCHEDULE", "pair")
+    pt = ss.FwiProblem(grid, n, src, rec_z, w, dev, torch.float32, 6, "tiled")
+    tiles = [(32, 8), (32, 16), (64, 4), (64, 8)]  # float4 x rows
+    combos = [(t, c, r, nt) for t in tiles for c in (1, 2, 3) for r in (0, 1) for nt in (0, 1)]
+    for i, ((tx, ty), c, r, nt) in enumerate(combos):
+        ss.clear_tuning()
+        ss.set_tuned_shape(grid, "fwi_pair",
+                           {"tile_x": 4 * tx, "tile_y": ty, "n_chunks": c, "xcd_remap": r, "nontemporal": nt})
+        ss.set_tuned_shape(grid, "fwi_recompute", {"tile_x": 64, "tile_y": 16, "n_chunks": 1 + i % 3})
+        assert pt.misfit(md, d_obs).item() == J_n
+        g, st = pt.gradient(md, src_trace=True)
+        assert np.array_equal(g.cpu().numpy(), g_n), combos[i]
+        assert np.array_equal(st.cpu().numpy(), st_n), combos[i]
+    ss.clear_tuning()
+
+
 @pytest.mark.parametrize("ckpt", [1, 24])
 def test_fwi_tune_without_overlap_timing(ss, dev, ckpt):
     """With one checkpoint per step, or one segment (every L p kept), the FWI
@@ -271,7 +338,8 @@ def test_gradient_replays_in_a_graph(ss, dev):
 
 
 def test_fwi_problem_tune(ss, dev):
-    """FwiProblem(tune=True) tunes both FWI kernels for its grid; its traces
+    """FwiProblem(tune=True) tunes the FWI kernels for its grid (forward,
+    adjoint, recompute beside the adjoint, and the step pair); its traces
     and gradient stay bit-identical to the global-memory kernels' (receiver
     plane written by the tiled step), also when tune() runs between misfit()
     and gradient()."""
@@ -284,8 +352,8 @@ def test_fwi_problem_tune(ss, dev):
     md = torch.from_numpy(m).to(dev)
     ss.clear_tuning()
     pt = ss.FwiProblem(grid, n, (18, 20, 52), 6, w, dev, torch.float32, tune=True)
-    assert set(pt.shape) == {"forward", "adjoint", "recompute"}
-    for kind in ("fwi_forward", "fwi_adjoint", "fwi_recompute"):
+    assert set(pt.shape) == {"forward", "adjoint", "recompute", "pair"}
+    for kind in ("fwi_forward", "fwi_adjoint", "fwi_recompute", "fwi_pair"):
         shp, hit = ss.tuned_shape(grid, kind)
         assert hit and shp["nontemporal"] in (0, 1)
         assert shp == pt.shape[kind[4:]]
