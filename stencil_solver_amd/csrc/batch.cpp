@@ -80,6 +80,7 @@ struct stencil_batch {
     bool ragged = true;
     bool persistent = false;  // ragged plan, one cooperative launch for all steps
     int persist_grid = 0;     // its workgroups
+    unsigned barrier_polls = kBarrierPolls;  // polls before a barrier times out
     unsigned* d_sync = nullptr;  // device: abort flag, barrier counter, 8 queues
     RaggedPlan plan;
     RaggedShot* d_shots = nullptr;  // device, one per shot
@@ -318,7 +319,7 @@ int build_graph(stencil_batch* b) {
                 if (b->persist_grid < 8) B_TRY(hipErrorNotSupported);
                 B_TRY(fill_f32(reinterpret_cast<float*>(b->d_sync), kSyncWords, 0.f, st));  // zero bits
                 B_TRY(ragged_persist_fwi_f32(b->d_shots, b->d_items, b->n_items, b->n_steps, b->plan.txv, b->plan.ty,
-                                             b->persist_grid, b->d_sync, st));
+                                             b->persist_grid, b->d_sync, b->barrier_polls, st));
             } else {
                 for (int n = 0; n < b->n_steps; ++n)
                     B_TRY(ragged_zq_fwi_f32(b->d_shots, b->d_items, b->n_items, n % 3, n, b->plan.txv, b->plan.ty,
@@ -410,6 +411,13 @@ int stencil_batch_create(const stencil_shot* shots, int n_shots, int n_steps, in
     b->n_steps = n_steps;
     b->ragged = !batch_mode_is("grouped");
     b->persistent = b->ragged && batch_mode_is("persistent");
+    // STENCIL_BATCH_BARRIER_POLLS: a test's way to make a persistent run's
+    // barriers time out (every workgroup still leaves at its next barrier)
+    if (const char* e = std::getenv("STENCIL_BATCH_BARRIER_POLLS")) {
+        char* end = nullptr;
+        const unsigned long v = std::strtoul(e, &end, 10);
+        if (end != e && *end == '\0' && v < kBarrierPolls) b->barrier_polls = unsigned(v);
+    }
     // group by grid (the grouped mode's launches; reported in either mode)
     using Key = std::tuple<uint64_t, uint64_t, uint64_t, double, double, double, double>;
     std::map<Key, int> index;

@@ -48,7 +48,7 @@ bool zq_supported(const Dims& d, const void* const* ptrs, int n_ptrs);
 // Whether (txv, ty) is a tile shape the library is built with.
 bool zq_shape_valid(int txv, int ty);
 // Shape-cache kinds: 0 wave step, 1 heat step, 2 + storage the 16-bit wave
-// step, then the two FWI kernels.
+// step, then the FWI kernels (forward, adjoint, recompute, step pair).
 constexpr int kKindFwiForward = 6;
 constexpr int kKindFwiAdjoint = 7;
 constexpr int kKindFwiRecompute = 8;  // forward steps beside the adjoint steps
@@ -158,10 +158,12 @@ hipError_t ragged_zq_fwi_f32(const RaggedShot* shots, const RaggedItem* items, i
 // grid barrier between steps; step n as ragged_zq_fwi_f32(..., n % 3, n, ...)
 // with non-temporal streams.  sync: 10 zeroed words (abort flag, barrier
 // counter, 8 queues); the flag is non-zero after the run when a barrier
-// timed out (the fields are then not valid).
+// timed out (the fields are then not valid).  A workgroup gives up waiting at
+// a barrier after max_polls polls (kBarrierPolls: seconds).
+constexpr unsigned kBarrierPolls = 1u << 22;
 int ragged_persist_grid(int txv, int ty);
 hipError_t ragged_persist_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int n_steps, int txv,
-                                  int ty, int grid, unsigned* sync, hipStream_t st);
+                                  int ty, int grid, unsigned* sync, unsigned max_polls, hipStream_t st);
 // The forward FWI step of n_shots shots of equal dims in one launch: shot i
 // steps ring[rot] / ring[rot+1] -> ring[rot+2] (mod 3), adding wavelet[step].
 hipError_t batch_zq_fwi_f32(const BatchShot* table, int n_shots, int rot, int step, const Dims& d,

@@ -877,8 +877,9 @@ void zq_kernel(ZqArgs a) {
 // and one empty slot to each of its G / 8 workgroups, so step n's slots start
 // at n (run_len + G / 8); the barrier counter reaches (n + 1) G when every
 // workgroup has finished step n.  One lane per workgroup spins, sleeping
-// between polls, for at most kBarrierPolls polls (seconds; a step takes
-// milliseconds): the host launches no more workgroups than stay resident
+// between polls, for at most max_polls polls (by default kBarrierPolls,
+// seconds; a step takes milliseconds): the host launches no more workgroups
+// than stay resident
 // together, and a barrier that still times out sets the abort flag, which
 // makes every workgroup leave at its next barrier, so the kernel always
 // ends.  The fences order this workgroup's stores of the step before its
@@ -887,9 +888,7 @@ void zq_kernel(ZqArgs a) {
 //
 // sync: [0] abort flag, [1] barrier counter, [2 + r] queue of run r; zero
 // first.
-constexpr unsigned kBarrierPolls = 1u << 22;
-
-__device__ bool grid_barrier(unsigned* sync, unsigned target) {
+__device__ bool grid_barrier(unsigned* sync, unsigned target, unsigned max_polls) {
     __shared__ int s_ok;
     __syncthreads();  // every lane's stores of the step issued
     if (threadIdx.x == 0) {
@@ -898,7 +897,7 @@ __device__ bool grid_barrier(unsigned* sync, unsigned target) {
         int ok = 1;
         for (unsigned polls = 0; __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
              ++polls) {
-            if (polls >= kBarrierPolls ||
+            if (polls >= max_polls ||
                 ((polls & 63u) == 0 && __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
@@ -916,7 +915,7 @@ __device__ bool grid_barrier(unsigned* sync, unsigned target) {
 template <int TXV, int TY, bool NT>
 __global__ __launch_bounds__(TXV* TY) __attribute__((amdgpu_waves_per_eu(min_waves(TXV, TY, dev::kF32, kWaveFwi))))
 void zq_persist_kernel(const RaggedShot* shots, const RaggedItem* items, int n_steps, int n_items, int alternate,
-                       unsigned* sync) {
+                       unsigned* sync, unsigned max_polls) {
     __shared__ int s_item;
     const unsigned run = blockIdx.x % 8u;
     const unsigned run_len = (static_cast<unsigned>(n_items) + 7u) / 8u;
@@ -943,12 +942,12 @@ void zq_persist_kernel(const RaggedShot* shots, const RaggedItem* items, int n_s
             if (item < 0) break;
             if (item < n_items) zq_tile<TXV, TY, kWaveFwi, NT>(s, static_cast<unsigned>(item));
         }
-        if (n + 1 < n_steps && !grid_barrier(sync, static_cast<unsigned>(n + 1) * gridDim.x)) return;
+        if (n + 1 < n_steps && !grid_barrier(sync, static_cast<unsigned>(n + 1) * gridDim.x, max_polls)) return;
     }
 }
 
 using KernelFn = void (*)(ZqArgs);
-using PersistFn = void (*)(const RaggedShot*, const RaggedItem*, int, int, int, unsigned*);
+using PersistFn = void (*)(const RaggedShot*, const RaggedItem*, int, int, int, unsigned*, unsigned);
 
 template <bool NT>
 PersistFn pick_persist(int txv, int ty) {
@@ -1301,7 +1300,7 @@ int ragged_persist_grid(int txv, int ty) {
 }
 
 hipError_t ragged_persist_fwi_f32(const RaggedShot* shots, const RaggedItem* items, int n_items, int n_steps, int txv,
-                                  int ty, int grid, unsigned* sync, hipStream_t st) {
+                                  int ty, int grid, unsigned* sync, unsigned max_polls, hipStream_t st) {
     if (n_items <= 0 || n_steps <= 0) return hipSuccess;
     PersistFn fn = pick_persist<true>(txv, ty);
     // the queues ((run_len + grid / 8) slots per step) and the barrier
@@ -1311,7 +1310,7 @@ hipError_t ragged_persist_fwi_f32(const RaggedShot* shots, const RaggedItem* ite
         return hipErrorInvalidValue;
     const int alternate = sweep_alternation() ? 1 : 0;
     hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(grid)), dim3(static_cast<unsigned>(txv * ty)), 0, st, shots,
-                       items, n_steps, n_items, alternate, sync);
+                       items, n_steps, n_items, alternate, sync, max_polls);
     return hipGetLastError();
 }
 

@@ -66,6 +66,34 @@ def test_batch_matches_oracle_per_shot(ss, dev, oracle, mode):
     b.close()
 
 
+def test_persistent_barrier_timeout_is_reported(ss, dev, oracle, monkeypatch):
+    """A persistent run whose barriers give up at their first poll
+    (STENCIL_BATCH_BARRIER_POLLS=0, a limit set from the host) still ends --
+    every workgroup leaves at its next barrier -- and check() reports the
+    timeout; the same shots in a batch made without the limit, run after it,
+    match the oracle."""
+    from stencil_solver_amd._lib import StencilError
+
+    n = 12
+    shots, host = _shots(ss, dev, n)
+    monkeypatch.setenv("STENCIL_BATCH_MODE", "persistent")
+    monkeypatch.setenv("STENCIL_BATCH_BARRIER_POLLS", "0")
+    b = ss.ShotBatch(shots, n, dev, tune=False)
+    assert b.persistent_grid >= 8
+    b.run()
+    with pytest.raises(StencilError, match="timed out|TIMEOUT|time"):
+        b.check()
+    b.close()
+    monkeypatch.delenv("STENCIL_BATCH_BARRIER_POLLS")
+    b = ss.ShotBatch(shots, n, dev, tune=False)
+    b.run()
+    b.check()
+    for i, (m, w, src) in enumerate(host):
+        want = oracle.shot_run(m, np.float32(10.0), np.float32(1e-3), n, src, w)
+        assert np.array_equal(b.newest(i).cpu().numpy(), want), i
+    b.close()
+
+
 def test_batch_rerun_and_tuned_shapes_are_identical(ss, dev, mode):
     n = 25
     shots, _ = _shots(ss, dev, n, seed=3)
