@@ -217,7 +217,7 @@ std::unique_ptr<BasicISolver<T>> make_basic_solver(const BasicConfig<T>& cfg) {
             solver = std::make_unique<HeatDiffusionSolver<T>>();
             break;
         default:
-            throw std::invalid_argument("make_solver: unknown Physics enum value");
+            throw std::invalid_argument("make_solver: cfg.physics is neither wave propagation nor heat diffusion");
     }
     solver->init(cfg);
     return solver;
