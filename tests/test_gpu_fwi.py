@@ -228,14 +228,15 @@ def _naive_reference(ss, dev, shape, n, ckpt, seed):
 
 
 @pytest.mark.parametrize("schedule", ["pair", "overlap", "serial"])
-@pytest.mark.parametrize("ckpt", [7, 1])
+@pytest.mark.parametrize("ckpt", [7, 1, 39])
 def test_backward_schedules_equal_naive(ss, dev, monkeypatch, schedule, ckpt):
     """The tiled FP32 backward pass in each schedule (STENCIL_FWI_SCHEDULE):
     the step pair (a recompute step and an adjoint step in one launch), the
     recompute on a second stream (the default), and one stream in turn.  All
     give the global-memory kernels' misfit, gradient and source trace bit for
     bit.  40 steps in segments of 7 (the short last segment leaves the first
-    paired round with single recompute steps) or of 1."""
+    paired round with single recompute steps), of 1, or of 39 (two segments,
+    the last one step long: one pair, then 38 single recompute steps)."""
     import torch
 
     shape, n = (36, 40, 104), 40
