@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-shot timing")
     ap.add_argument("--batch-shots", type=int, default=64, help="shots of the batched modelling")
     ap.add_argument("--batch-steps", type=int, default=1500, help="steps of the batched modelling")
+    ap.add_argument("--batch-plan", default=None, metavar="TILE_X,TILE_Y,CHUNK,XCD_RUNS",
+                    help="fixed ragged-batch plan (cells, rows, z-chunk planes, 0/1) instead of the batch tuner's, "
+                         "for both the per-step and the persistent run (profiles fix it so every pass runs one plan)")
     ap.add_argument("--fwi-n", type=int, default=256, help="FWI grid cells per axis")
     ap.add_argument("--fwi-steps", type=int, default=2000, help="FWI time steps")
     ap.add_argument("--calibrate", action="store_true",
@@ -346,7 +349,8 @@ def main():
     if not args.no_fwi:
         result["fwi"] = fwi_study(ss, args.fwi_n, args.fwi_steps, dev, tune=not args.no_tune)
     if not args.no_batch:
-        result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev, tune=not args.no_tune)
+        plan = tuple(int(v) for v in args.batch_plan.split(",")) if args.batch_plan else None
+        result["batch"] = batch_study(ss, args.batch_shots, args.batch_steps, dev, tune=not args.no_tune, plan=plan)
 
     if args.calibrate:
         # Known-size copy: n^3 floats read + n^3 floats written per launch, the
@@ -470,14 +474,16 @@ def precision_study(ss, p0, vel2, n, steps, tune=True):
     return out
 
 
-def batch_study(ss, n_shots, steps, dev, tune=True):
+def batch_study(ss, n_shots, steps, dev, tune=True, plan=None):
     """Batched multi-shot modelling: n_shots cubic grids with n drawn uniformly
     from {96, 128, 160, 192, 256, 320} (seed 4); each shot: seeded smooth
     velocity (Gaussian-filtered noise, sigma 8 cells) mapped to [1500, 4500]
     m/s, h = 10 m, dt = 8e-4 s, 15 Hz Ricker at a random interior cell; FP32.
     Launch shapes autotuned per grid size; the run is one hipGraph launch.
     Then the same run as one persistent launch (STENCIL_BATCH_MODE=persistent),
-    timed the same way and compared shot by shot for bit-equality."""
+    timed the same way and compared shot by shot for bit-equality.  plan:
+    (tile_x, tile_y, chunk_planes, xcd_runs) fixed for both runs instead of
+    tuning."""
     import numpy as np
     import torch
     from stencil_solver_amd import synth
@@ -497,7 +503,9 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
     t_inputs = time.time() - t0
     print(f"[bench] batch inputs: {n_shots} shots in {t_inputs:.1f}s", file=sys.stderr, flush=True)
     t0 = time.time()
-    b = ss.ShotBatch(shots, steps, dev, tune=tune)
+    b = ss.ShotBatch(shots, steps, dev, tune=tune and plan is None)
+    if plan is not None:
+        b.set_plan(plan[0], plan[1], plan[2], bool(plan[3]))
     torch.cuda.synchronize()
     t_build = time.time() - t0
     print(f"[bench] batch graph built in {t_build:.1f}s", file=sys.stderr, flush=True)
@@ -527,7 +535,9 @@ def batch_study(ss, n_shots, steps, dev, tune=True):
         b.close()
         os.environ["STENCIL_BATCH_MODE"] = "persistent"
         try:
-            bp = ss.ShotBatch(shots, steps, dev, tune=tune)
+            bp = ss.ShotBatch(shots, steps, dev, tune=tune and plan is None)
+            if plan is not None:
+                bp.set_plan(plan[0], plan[1], plan[2], bool(plan[3]))
         finally:
             del os.environ["STENCIL_BATCH_MODE"]
         bp.run()
