@@ -74,7 +74,7 @@ std::vector<ZqShape> zq_tune_candidates(const Dims& d) {
     const int64_t inz = d.nz - 8;
     std::vector<int> chunks;
     for (int c : {1, 2, 4, 8, 16})
-        if (c == 1 || inz / c >= 16) chunks.push_back(c);
+        if (c == 1 || inz / c >= 9) chunks.push_back(c);
     std::vector<ZqShape> out;
     // xcd_remap 0 / 1: workgroup order as dispatched, or each XCD a
     // contiguous range of tiles.
