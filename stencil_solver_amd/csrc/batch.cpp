@@ -253,7 +253,7 @@ hipError_t zero_rings(const stencil_batch* b, hipStream_t st) {
 // buffers (zeroed first; the run starts from zero again anyway).  A sweep
 // timing carries a few per cent of noise, enough to reorder plans within a
 // few per cent of each other: the four fastest plans are timed again, twice
-// each with at least 12 launches, and each keeps its best time.
+// each with at least 32 launches, and each keeps its best time.
 hipError_t tune_plan(stencil_batch* b, int reps, hipStream_t st) {
     static const int shapes[][2] = {{16, 16}, {16, 32}, {32, 8}, {32, 16}, {64, 4}, {64, 8}, {64, 12}, {64, 16}};
     hipError_t e = zero_rings(b, st);
@@ -316,7 +316,7 @@ hipError_t tune_plan(stencil_batch* b, int reps, hipStream_t st) {
     for (int round = 0; round < 2; ++round)
         for (Row& r : rows) {
             float ms = -1.f;
-            e = time_plan(r.plan, std::max(3 * reps, 12), &ms);
+            e = time_plan(r.plan, std::max(8 * reps, 32), &ms);
             if (e != hipSuccess) return e;
             if (ms < r.ms) r.ms = ms;
         }
