@@ -107,7 +107,6 @@ struct Plan {
     size_t field = 0;  // bytes from one field of the workspace to the next
 };
 
-
 bool make_plan(const stencil_grid* g, const stencil_fwi_config* c, size_t esz, Plan* p) {
     if (!g || !c || g->nx <= 8 || g->ny <= 8 || g->nz <= 8 || c->n_steps == 0) return false;
     if (g->nx > (1u << 20) || g->ny > (1u << 20) || g->nz > (1u << 20)) return false;
@@ -307,6 +306,16 @@ int forward(const T* m, const stencil_grid* g, const stencil_fwi_config* c, cons
 // the older switch, means serial).  All three give the same bits.
 enum class Schedule { kSerial, kOverlap, kPair };
 
+Schedule fwi_schedule() {
+    const char* e = std::getenv("STENCIL_FWI_SCHEDULE");
+    if (e && std::strcmp(e, "serial") == 0) return Schedule::kSerial;
+    if (e && std::strcmp(e, "overlap") == 0) return Schedule::kOverlap;
+    if (e && std::strcmp(e, "pair") == 0) return Schedule::kPair;
+    const char* o = std::getenv("STENCIL_FWI_OVERLAP");
+    if (o && o[0] == '0') return Schedule::kSerial;
+    return Schedule::kOverlap;
+}
+
 // Whether the step pair can take this backward call: FP32 on the tiled kernel
 // with dims and buffers it accepts (the workspace's fields are 256-byte
 // aligned offsets of an aligned base, and its receiver planes 16-byte
@@ -316,15 +325,6 @@ bool pair_ok(bool tiled, const Plan& p, const T* m, const T* grad, const void* w
     if (!std::is_same<T, float>::value || !tiled) return false;
     const void* ptrs[3] = {m, grad, work};
     return zq_supported(p.d, ptrs, 3);
-}
-Schedule fwi_schedule() {
-    const char* e = std::getenv("STENCIL_FWI_SCHEDULE");
-    if (e && std::strcmp(e, "serial") == 0) return Schedule::kSerial;
-    if (e && std::strcmp(e, "overlap") == 0) return Schedule::kOverlap;
-    if (e && std::strcmp(e, "pair") == 0) return Schedule::kPair;
-    const char* o = std::getenv("STENCIL_FWI_OVERLAP");
-    if (o && o[0] == '0') return Schedule::kSerial;
-    return Schedule::kOverlap;
 }
 
 // The recompute stream of the current device: created once (default
